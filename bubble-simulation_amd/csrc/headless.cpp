@@ -1,0 +1,304 @@
+// headless.cpp -- `bubble-sim`, the command-line front end.
+//
+// Keeps the reference's flags (config.rs:13-115) and its headless report
+// (headless.rs:23-116) byte for byte where the numbers agree, on top of the C
+// ABI.  Extensions: --precision, --math, --kernel, --bubble-at-centre,
+// --nucleation uniform-steps with --uniform-max-step/--time-seed, --dump.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bubblesim.h"
+
+namespace {
+
+// Rust's `{:.Ne}` (LowerExp): mantissa digits as printf, exponent without
+// '+' and without leading zeros ("1.229e-2", "0.000e0").
+std::string rust_exp(double v, int prec) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*e", prec, v);
+    std::string s(buf);
+    const size_t e = s.find('e');
+    if (e == std::string::npos) return s;
+    const int ex = std::atoi(s.c_str() + e + 1);
+    return s.substr(0, e) + "e" + std::to_string(ex);
+}
+
+std::string rpad(const std::string &s, size_t w) { return s.size() >= w ? s : std::string(w - s.size(), ' ') + s; }
+
+void usage(FILE *f) {
+    std::fprintf(f,
+                 "Live GPU lattice simulation of bubble nucleation and growth in a first-order cosmological phase "
+                 "transition.\n\n"
+                 "Usage: bubble-sim [OPTIONS]\n\n"
+                 "Options:\n"
+                 "      --grid <GRID>                  Lattice size: N or NX,NY,NZ [default: 192]\n"
+                 "      --lambda <LAMBDA>              Quartic coupling [default: 0.5]\n"
+                 "      --eps-ratio <EPS_RATIO>        Vacuum energy split [default: 0.3]\n"
+                 "      --alpha <ALPHA>                Transition strength [default: 0.1]\n"
+                 "      --eta <ETA>                    Wall friction [default: 0.3]\n"
+                 "      --cfl <CFL>                    Courant number [default: 0.2]\n"
+                 "      --seed-factor <SEED_FACTOR>    Seed radius in units of R_c [default: 1.15]\n"
+                 "      --seed-radius <CELLS>          Seed radius in cells (overrides --seed-factor)\n"
+                 "      --bubbles <BUBBLES>            Number of bubbles [default: 12]\n"
+                 "      --nucleation <NUCLEATION>      simultaneous | exponential | uniform-steps [default: exponential]\n"
+                 "      --nucleation-duration <T>      Time over which exponential nucleation completes [default: 60]\n"
+                 "      --seed <SEED>                  Seed for bubble placement [default: 1]\n"
+                 "      --steps-per-frame <N>          (interactive only) [default: 2]\n"
+                 "      --field <FIELD>                (interactive only) [default: kinetic]\n"
+                 "      --no-vsync                     (interactive only)\n"
+                 "      --headless <STEPS>             Run this many steps without a window, printing diagnostics\n"
+                 "      --report-every <N>             Report interval in headless mode [default: steps/20]\n"
+                 "      --screenshot <PATH>            (rendering is not part of this build)\n"
+                 "      --screenshot-size <WxH>        [default: 1280x960]\n"
+                 "  extensions:\n"
+                 "      --precision <f32|f64>          Lattice precision [default: f32]\n"
+                 "      --math <fast|exact>            Arithmetic policy of the stage kernels [default: fast]\n"
+                 "      --kernel <fused|two-pass>      Stage implementation [default: fused]\n"
+                 "      --bubble-at-centre             One bubble at the box centre at t = 0\n"
+                 "      --uniform-max-step <N>         uniform-steps nucleation: steps in [0, N] [default: 200]\n"
+                 "      --time-seed <SEED>             uniform-steps nucleation time generator seed\n"
+                 "      --dump <PATH>                  Write the final state (field then fluid, raw) to PATH\n"
+                 "  -h, --help                         Print help\n"
+                 "  -V, --version                      Print version\n");
+}
+
+bool parse_f32(const std::string &v, float &out) {
+    char *end = nullptr;
+    out = std::strtof(v.c_str(), &end);
+    return end && *end == '\0' && !v.empty();
+}
+bool parse_u64(const std::string &v, uint64_t &out) {
+    if (v.empty() || v.find_first_not_of("0123456789") != std::string::npos) return false;
+    out = std::strtoull(v.c_str(), nullptr, 10);
+    return true;
+}
+
+int fail_usage(const std::string &msg) {
+    std::fprintf(stderr, "error: %s\n\nFor more information, try '--help'.\n", msg.c_str());
+    return 2;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    bs_spec spec;
+    bs_spec_default(&spec);
+    float seed_factor = 1.15f;
+    bool have_seed_radius = false;
+    float seed_radius = 0.0f;
+    bool have_headless = false, have_screenshot = false;
+    uint64_t headless = 0, report_every = 0;
+    bool have_report = false;
+    uint64_t steps_per_frame = 2;
+    std::string dump_path;
+
+    std::vector<std::string> args(argv + 1, argv + argc);
+    for (size_t i = 0; i < args.size(); ++i) {
+        std::string a = args[i], val;
+        bool has_inline = false;
+        const size_t eq = a.find('=');
+        if (a.rfind("--", 0) == 0 && eq != std::string::npos) {
+            val = a.substr(eq + 1);
+            a = a.substr(0, eq);
+            has_inline = true;
+        }
+        auto need = [&](std::string &v) -> bool {
+            if (has_inline) {
+                v = val;
+                return true;
+            }
+            if (i + 1 >= args.size()) return false;
+            v = args[++i];
+            return true;
+        };
+        std::string v;
+        if (a == "-h" || a == "--help") {
+            usage(stdout);
+            return 0;
+        } else if (a == "-V" || a == "--version") {
+            std::printf("bubble-sim %s\n", bs_version());
+            return 0;
+        } else if (a == "--no-vsync") {
+        } else if (a == "--bubble-at-centre") {
+            spec.bubble_at_centre = 1;
+            spec.bubbles = 1;
+        } else if (!need(v)) {
+            if (a.rfind("-", 0) == 0) return fail_usage("a value is required for '" + a + "' but none was supplied");
+            return fail_usage("unexpected argument '" + a + "' found");
+        } else if (a == "--grid") {
+            if (bs_parse_grid(v.c_str(), spec.grid) != BS_OK)
+                return fail_usage("invalid value '" + v + "' for '--grid <GRID>': " + bs_last_error());
+        } else if (a == "--lambda" || a == "--eps-ratio" || a == "--alpha" || a == "--eta" || a == "--cfl" ||
+                   a == "--seed-factor" || a == "--seed-radius" || a == "--nucleation-duration") {
+            float f;
+            if (!parse_f32(v, f)) return fail_usage("invalid value '" + v + "' for '" + a + "': invalid float literal");
+            if (a == "--lambda") spec.model.lambda = f;
+            else if (a == "--eps-ratio") spec.model.eps_ratio = f;
+            else if (a == "--alpha") spec.model.alpha = f;
+            else if (a == "--eta") spec.model.eta = f;
+            else if (a == "--cfl") spec.model.cfl = f;
+            else if (a == "--seed-factor") seed_factor = f;
+            else if (a == "--seed-radius") { have_seed_radius = true; seed_radius = f; }
+            else spec.nucleation_duration = f;
+        } else if (a == "--bubbles" || a == "--seed" || a == "--steps-per-frame" || a == "--headless" ||
+                   a == "--report-every" || a == "--uniform-max-step" || a == "--time-seed") {
+            uint64_t u;
+            if (!parse_u64(v, u)) return fail_usage("invalid value '" + v + "' for '" + a + "': invalid digit found in string");
+            if (a == "--bubbles") spec.bubbles = (uint32_t)u;
+            else if (a == "--seed") spec.seed = u;
+            else if (a == "--steps-per-frame") steps_per_frame = u;
+            else if (a == "--headless") { have_headless = true; headless = u; }
+            else if (a == "--report-every") { have_report = true; report_every = u; }
+            else if (a == "--uniform-max-step") spec.uniform_max_step = (uint32_t)u;
+            else spec.time_seed = u;
+        } else if (a == "--nucleation") {
+            if (v == "simultaneous") spec.nucleation = BS_NUC_SIMULTANEOUS;
+            else if (v == "exponential") spec.nucleation = BS_NUC_EXPONENTIAL;
+            else if (v == "uniform-steps") spec.nucleation = BS_NUC_UNIFORM_STEPS;
+            else return fail_usage("invalid value '" + v + "' for '--nucleation <NUCLEATION>'");
+        } else if (a == "--field") {
+            if (v != "phi" && v != "speed" && v != "temperature" && v != "kinetic")
+                return fail_usage("invalid value '" + v + "' for '--field <FIELD>'");
+        } else if (a == "--screenshot") {
+            have_screenshot = true;
+        } else if (a == "--screenshot-size") {
+        } else if (a == "--precision") {
+            if (v == "f32") spec.precision = BS_PRECISION_F32;
+            else if (v == "f64") spec.precision = BS_PRECISION_F64;
+            else return fail_usage("invalid value '" + v + "' for '--precision'");
+        } else if (a == "--math") {
+            if (v == "fast") spec.math = BS_MATH_FAST;
+            else if (v == "exact") spec.math = BS_MATH_EXACT;
+            else return fail_usage("invalid value '" + v + "' for '--math'");
+        } else if (a == "--kernel") {
+            if (v == "fused") spec.kernel = BS_KERNEL_FUSED;
+            else if (v == "two-pass") spec.kernel = BS_KERNEL_TWO_PASS;
+            else return fail_usage("invalid value '" + v + "' for '--kernel'");
+        } else if (a == "--dump") {
+            dump_path = v;
+        } else {
+            return fail_usage("unexpected argument '" + a + "' found");
+        }
+    }
+    if (spec.precision == BS_PRECISION_F64) spec.math = BS_MATH_EXACT;
+    spec.model.seed_kind = have_seed_radius ? BS_SEED_CELLS : BS_SEED_CRITICAL;
+    spec.model.seed_value = have_seed_radius ? seed_radius : seed_factor;
+
+    // Config::validate (config.rs:151-178)
+    if (bs_spec_validate(&spec) != BS_OK) {
+        std::fprintf(stderr, "Error: %s\n", bs_last_error());
+        return 1;
+    }
+    if (steps_per_frame == 0) {
+        std::fprintf(stderr, "Error: --steps-per-frame must be at least 1\n");
+        return 1;
+    }
+    if (have_screenshot) {
+        std::fprintf(stderr, "Error: --screenshot needs the volume renderer, which is not part of this build\n");
+        return 1;
+    }
+    if (!have_headless) {
+        std::fprintf(stderr, "Error: the interactive window is not part of this build; run with --headless <STEPS>\n");
+        return 1;
+    }
+    const uint64_t steps = headless;
+    const uint64_t every = have_report ? report_every : (steps / 20 > 1 ? steps / 20 : 1);  // main.rs:32
+    if (every == 0) {
+        std::fprintf(stderr, "Error: --report-every must be at least 1\n");
+        return 1;
+    }
+
+    char name[256] = "unknown";
+    if (bs_device_name(name, sizeof name) != BS_OK) {
+        std::fprintf(stderr, "Error: no suitable GPU adapter found: %s\n", bs_last_error());
+        return 1;
+    }
+    std::printf("adapter: %s (HIP)\n", name);
+
+    bs_sim *sim = nullptr;
+    if (bs_sim_create(&spec, &sim) != BS_OK) {
+        std::fprintf(stderr, "Error: failed to create the simulation\n\nCaused by:\n    %s\n", bs_last_error());
+        return 1;
+    }
+    bs_derived d;
+    bs_model_derive(&spec.model, &d);
+    std::printf("lattice %ux%ux%u = %.2f M cells | %.2f GB | dt = %.3f\n", spec.grid[0], spec.grid[1], spec.grid[2],
+                (double)bs_sim_cell_count(sim) / 1e6, (double)bs_sim_device_memory_bytes(sim) / 1e9, (double)d.dt);
+    std::printf("wall %.2f cells | R_c %.1f cells | sigma %.4f | eps %.4f | e_ref %.4f\n", (double)d.wall_width,
+                (double)d.critical_radius, (double)d.surface_tension, (double)d.eps, (double)d.e_ref);
+    if (steps > 0) {
+        std::printf("\n%10s %10s %10s %10s %11s %11s %10s\n", "step", "time", "broken", "R_eq", "max|v|", "K", "E/E0");
+    }
+    std::fflush(stdout);
+
+    bool have_base = false;
+    double base = 0.0;
+    uint64_t done = 0;
+    const auto start = std::chrono::steady_clock::now();
+    int rc = 0;
+    while (done < steps) {
+        const uint64_t chunk = every < steps - done ? every : steps - done;
+        if (bs_sim_advance(sim, chunk) != BS_OK) {
+            std::fprintf(stderr, "Error: %s\n", bs_last_error());
+            rc = 1;
+            break;
+        }
+        bs_diagnostics diag;
+        if (bs_sim_diagnostics(sim, &diag) != BS_OK) {
+            std::fprintf(stderr, "Error: %s\n", bs_last_error());
+            rc = 1;
+            break;
+        }
+        done += chunk;
+        if (diag.nonfinite || !std::isfinite(diag.mean_energy) || !std::isfinite(diag.max_velocity)) {
+            std::fprintf(stderr,
+                         "Error: the solution blew up at step %llu (non-finite diagnostics); reduce --cfl or --alpha\n",
+                         (unsigned long long)done);
+            rc = 1;
+            break;
+        }
+        if (!have_base && bs_sim_remaining_bubbles(sim) == 0) {
+            have_base = true;
+            base = diag.mean_energy;
+        }
+        char ratio[64];
+        if (have_base) std::snprintf(ratio, sizeof ratio, "%.6f", diag.mean_energy / base);
+        else std::snprintf(ratio, sizeof ratio, "-");
+        const float cells = (float)bs_sim_cell_count(sim);
+        const float volume = (float)diag.broken_fraction * cells;
+        const float r_eq = std::cbrt(volume * 3.0f / (4.0f * 3.14159265358979323846f));
+        std::printf("%10llu %10.1f %9.2f%% %10.1f %11.4f %s %10s\n", (unsigned long long)done, (double)bs_sim_time(sim),
+                    100.0 * diag.broken_fraction, (double)r_eq, diag.max_velocity,
+                    rpad(rust_exp(diag.kinetic_fraction, 3), 11).c_str(), ratio);
+        std::fflush(stdout);
+    }
+    const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    if (!rc && steps > 0) {
+        const double cell_steps = (double)steps * (double)bs_sim_cell_count(sim);
+        std::printf("\n%llu steps in %.2f s  (%.1f steps/s, %.2f G cell-updates/s)\n", (unsigned long long)steps,
+                    elapsed, (double)steps / elapsed, cell_steps / elapsed / 1e9);
+    }
+    if (!rc && !dump_path.empty()) {
+        const size_t real = spec.precision == BS_PRECISION_F64 ? 8 : 4;
+        const uint64_t n = bs_sim_cell_count(sim);
+        std::vector<unsigned char> buf(n * 6 * real);
+        if (bs_sim_download(sim, buf.data(), buf.data() + n * 2 * real) != BS_OK) {
+            std::fprintf(stderr, "Error: %s\n", bs_last_error());
+            rc = 1;
+        } else {
+            FILE *f = std::fopen(dump_path.c_str(), "wb");
+            if (!f || std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+                std::fprintf(stderr, "Error: cannot write %s\n", dump_path.c_str());
+                rc = 1;
+            }
+            if (f) std::fclose(f);
+        }
+    }
+    bs_sim_destroy(sim);
+    return rc;
+}

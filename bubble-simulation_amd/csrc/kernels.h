@@ -1,0 +1,477 @@
+// kernels.h -- HIP kernels of the lattice solver for gfx950 (wave64).
+//
+// Kernel set (reference entry point each replaces, shaders/*.wgsl):
+//   k_init            init.wgsl:18-26
+//   k_stamp           nucleate.wgsl:59-85 (bubble batch passed by value)
+//   k_prim            primitives.wgsl:16-23        (two-pass path only)
+//   k_stage_twopass   step.wgsl:92-174             (two-pass path only)
+//   k_stage_fused     primitives + step fused, one launch per RK stage
+//   k_reduce1/2       reduce.wgsl:81-140, FP64 compensated
+//
+// State is AoS: field V2 (phi, pi), fluid V4 (E, Zx, Zy, Zz), x fastest,
+// i = (z*ny + y)*nx + x (common.wgsl:91-95).
+#pragma once
+
+#include "scheme.h"
+
+namespace bs {
+
+// One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
+// three RK slots; `cur` is the slot holding U^n (sim.rs:180-242 `cur`).
+template <class R> struct Box {
+    Params<R> P;
+    V2<R> *field[3];
+    V4<R> *fluid[3];
+    int cur;
+    int pad[3];
+};
+
+// stage_slots (sim.rs:1046-1051) for stage s: a=cur, b=cur+1, c=cur+2;
+// (in, out) = (a,b), (b,c), (c,b); U^n is always a.
+__host__ __device__ inline void stage_slot_ids(int cur, int s, int &in, int &out) {
+    const int a = cur, b = (cur + 1) % 3, c = (cur + 2) % 3;
+    in = s == 0 ? a : (s == 1 ? b : c);
+    out = s == 1 ? c : b;
+}
+
+template <class R> struct RK3c;
+template <> struct RK3c<float> {
+    // RK3 (sim.rs:135)
+    __host__ __device__ static inline void get(int s, float &a0, float &a1) {
+        a0 = s == 0 ? 0.0f : (s == 1 ? 0.75f : 1.0f / 3.0f);
+        a1 = s == 0 ? 1.0f : (s == 1 ? 0.25f : 2.0f / 3.0f);
+    }
+};
+template <> struct RK3c<double> {
+    // the f32 coefficients, widened (the FP64 scheme shares every constant
+    // that the reference stores as f32)
+    __host__ __device__ static inline void get(int s, double &a0, double &a1) {
+        a0 = s == 0 ? 0.0 : (s == 1 ? 0.75 : (double)(1.0f / 3.0f));
+        a1 = s == 0 ? 1.0 : (s == 1 ? 0.25 : (double)(2.0f / 3.0f));
+    }
+};
+
+// ---------------------------------------------------------------------------
+//  init (init.wgsl:18-26)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256) k_init(V2<R> *field, V4<R> *fluid, int64_t n, R e_ref) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    field[i] = V2<R>{R(0), R(0)};
+    fluid[i] = V4<R>{e_ref, R(0), R(0), R(0)};
+}
+
+// ---------------------------------------------------------------------------
+//  nucleate (nucleate.wgsl:59-85): phi = max(phi, profile_b) over a batch of
+//  bubbles; pi untouched.
+// ---------------------------------------------------------------------------
+constexpr int kStampBatch = 32;
+template <class R> struct StampBatch {
+    int count;
+    int pad[3];
+    R b[kStampBatch][4];  // cx, cy, cz, r0 (f32 values, widened for R = double)
+};
+
+template <class R>
+__global__ void __launch_bounds__(256) k_stamp(V2<R> *field, Params<R> P, StampBatch<R> batch) {
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % P.nx);
+    const int y = (int)((i / P.nx) % P.ny);
+    const int z = (int)(i / ((int64_t)P.nx * P.ny));
+    V2<R> f = field[i];
+    R phi = f.x;
+    for (int b = 0; b < batch.count; ++b) {
+        const R prof = bubble_profile<R>(x, y, z, batch.b[b][0], batch.b[b][1], batch.b[b][2], batch.b[b][3], P);
+        phi = ExactMath<R>::max(phi, prof);
+    }
+    field[i] = V2<R>{phi, f.y};
+}
+
+// ---------------------------------------------------------------------------
+//  Two-pass path: primitives (primitives.wgsl:16-23), then the stage
+//  (step.wgsl:92-174) reading the primitive cache.  Mirrors the reference's
+//  dispatch structure; faces are evaluated from both sides as there.
+// ---------------------------------------------------------------------------
+template <class M, class R>
+__global__ void __launch_bounds__(256) k_prim(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
+                                              V4<R> *__restrict__ prim, Params<R> P) {
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    prim[i] = eos_prim_from_cons<M>(fluid[i], field[i].x, P);
+}
+
+template <int A, class M, class R>
+__device__ inline void twopass_axis(const V2<R> *__restrict__ fin, const V4<R> *__restrict__ prim, const Params<R> &P,
+                                    int x, int y, int z, R phi0, const V4<R> &q0, R &lap_sum, R &g, V4<R> &flux_div) {
+    auto idx = [&](int dd) -> int64_t {
+        const int xx = A == 0 ? wrapi(x + dd, P.nx) : x;
+        const int yy = A == 1 ? wrapi(y + dd, P.ny) : y;
+        const int zz = A == 2 ? wrapi(z + dd, P.nz) : z;
+        return ((int64_t)zz * P.ny + yy) * P.nx + xx;
+    };
+    const int64_t im1 = idx(-1), ip1 = idx(1), im2 = idx(-2), ip2 = idx(2);
+    const R phi_m = fin[im1].x;
+    const R phi_p = fin[ip1].x;
+    lap_sum = lap_sum + (phi_m + phi_p - R(2) * phi0);
+    g = R(0.5) * (phi_p - phi_m) * P.inv_dx;
+    const V4<R> qm2 = sanitise<M>(prim[im2], P);
+    const V4<R> qm1 = sanitise<M>(prim[im1], P);
+    const V4<R> qp1 = sanitise<M>(prim[ip1], P);
+    const V4<R> qp2 = sanitise<M>(prim[ip2], P);
+    const V4<R> slope_m1 = limit_slope4<M>(sub4(qm1, qm2), sub4(q0, qm1));
+    const V4<R> slope_0 = limit_slope4<M>(sub4(q0, qm1), sub4(qp1, q0));
+    const V4<R> slope_p1 = limit_slope4<M>(sub4(qp1, q0), sub4(qp2, qp1));
+    const V4<R> f_minus = face_flux_sl<A, M>(qm1, slope_m1, q0, slope_0, phi_m, phi0, P);
+    const V4<R> f_plus = face_flux_sl<A, M>(q0, slope_0, qp1, slope_p1, phi0, phi_p, P);
+    flux_div.x = flux_div.x + (f_plus.x - f_minus.x) * P.inv_dx;
+    flux_div.y = flux_div.y + (f_plus.y - f_minus.y) * P.inv_dx;
+    flux_div.z = flux_div.z + (f_plus.z - f_minus.z) * P.inv_dx;
+    flux_div.w = flux_div.w + (f_plus.w - f_minus.w) * P.inv_dx;
+}
+
+template <class M, class R>
+__global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__ field_n, const V4<R> *__restrict__ fluid_n,
+                                                       const V2<R> *__restrict__ field_in, const V4<R> *__restrict__ fluid_in,
+                                                       const V4<R> *__restrict__ prim, V2<R> *__restrict__ field_out,
+                                                       V4<R> *__restrict__ fluid_out, Params<R> P, R a0, R a1) {
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 >= n) return;
+    const int x = (int)(i0 % P.nx);
+    const int y = (int)((i0 / P.nx) % P.ny);
+    const int z = (int)(i0 / ((int64_t)P.nx * P.ny));
+    const V2<R> f0 = field_in[i0];
+    const R phi0 = f0.x, pi0 = f0.y;
+    const V4<R> q0 = sanitise<M>(prim[i0], P);
+    R lap_sum = R(0), gx, gy, gz;
+    V4<R> flux_div{R(0), R(0), R(0), R(0)};
+    twopass_axis<0, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gx, flux_div);
+    twopass_axis<1, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gy, flux_div);
+    twopass_axis<2, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gz, flux_div);
+    const V2<R> fnn = field_n[i0];
+    R phi_o, pi_o;
+    V4<R> fo;
+    stage_update<M>(phi0, pi0, q0, lap_sum, gx, gy, gz, flux_div, fluid_in[i0], fnn.x, fnn.y, fluid_n[i0], a0, a1,
+                    P.dt, P, phi_o, pi_o, fo);
+    field_out[i0] = V2<R>{phi_o, pi_o};
+    fluid_out[i0] = fo;
+}
+
+// ---------------------------------------------------------------------------
+//  Fused stage: one workgroup owns a TX x TY column of cells over a z range
+//  [zb, ze) and marches up in z.  Per plane:
+//    - the sanitised primitives of the plane, with a 2-cell halo in x and y,
+//      sit in LDS (interior ones come from registers, where they were
+//      computed two planes earlier; halo ones are recomputed here);
+//    - every x- and y-face of the tile is evaluated once (face items are
+//      spread over all threads) and stored in LDS;
+//    - each thread evaluates its own +z face from a register ring of the
+//      column's primitives q(z..z+2), slope(z) and carries the -z face;
+//    - the divergence, friction and RK combination are applied per cell.
+//  The result is bitwise that of k_prim + k_stage_twopass under ExactMath:
+//  every face sees the same operands in the same order.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, class R> struct FusedLds {
+    V4<R> q[TY + 4][TX + 4];
+    R phi[TY + 2][TX + 2];
+    V4<R> fx[TY][TX + 1];
+    V4<R> fy[TY + 1][TX];
+};
+
+template <class R> __device__ inline int64_t cidx(const Params<R> &P, int x, int y, int z) {
+    return ((int64_t)z * P.ny + y) * P.nx + x;
+}
+
+template <int TX, int TY, class M, class R>
+__device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
+                                           const V2<R> *__restrict__ field_n, const V4<R> *__restrict__ fluid_n,
+                                           const V2<R> *__restrict__ field_in, const V4<R> *__restrict__ fluid_in,
+                                           V2<R> *__restrict__ field_out, V4<R> *__restrict__ fluid_out,
+                                           int x0, int y0, int zb, int ze, int stage) {
+    constexpr int NT = TX * TY;
+    const int tid = threadIdx.x;
+    const int tx = tid % TX;
+    const int ty = tid / TX;
+    const int gx = wrapi(x0 + tx, P.nx);
+    const int gy = wrapi(y0 + ty, P.ny);
+    const bool valid = (x0 + tx < P.nx) && (y0 + ty < P.ny);
+    R a0, a1;
+    RK3c<R>::get(stage, a0, a1);
+    const R dt = P.dt;
+
+    auto prim_at = [&](int64_t i, R &phi_out) -> V4<R> {
+        const V2<R> f = field_in[i];
+        phi_out = f.x;
+        return sanitise<M>(eos_prim_from_cons<M>(fluid_in[i], f.x, P), P);
+    };
+
+    // ---- prologue: column primitives for zb-2 .. zb+1, the face zb-1/2 ----
+    R ph_m, ph_0, ph_1, ph_tmp;
+    V4<R> qa = prim_at(cidx(P, gx, gy, wrapi(zb - 2, P.nz)), ph_tmp);
+    V4<R> qb = prim_at(cidx(P, gx, gy, wrapi(zb - 1, P.nz)), ph_m);
+    V4<R> qz = prim_at(cidx(P, gx, gy, wrapi(zb, P.nz)), ph_0);
+    V4<R> qz1 = prim_at(cidx(P, gx, gy, wrapi(zb + 1, P.nz)), ph_1);
+    const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
+    V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
+    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, ph_0, P);
+
+    // halo items of a plane: 4 TY x-halo cells, then 4 TX y-halo cells
+    constexpr int NH = 4 * (TX + TY);
+
+    for (int z = zb; z < ze; ++z) {
+        const int zp2 = wrapi(z + 2, P.nz);
+        // (a) plane z+2 of this column (for the +z face)
+        R ph_2;
+        const V4<R> q2 = prim_at(cidx(P, gx, gy, zp2), ph_2);
+
+        // (b) plane z into LDS: interior from registers, halo recomputed
+        L.q[ty + 2][tx + 2] = qz;
+        L.phi[ty + 1][tx + 1] = ph_0;
+        for (int h = tid; h < NH; h += NT) {
+            int hx, hy;
+            if (h < 4 * TY) {
+                const int r = h / 4, c = h % 4;
+                hy = r;
+                hx = c < 2 ? c - 2 : TX + (c - 2);
+            } else {
+                const int k = h - 4 * TY;
+                const int r = k / TX, c = k % TX;
+                hx = c;
+                hy = r < 2 ? r - 2 : TY + (r - 2);
+            }
+            R phh;
+            const V4<R> qh = prim_at(cidx(P, wrapi(x0 + hx, P.nx), wrapi(y0 + hy, P.ny), z), phh);
+            L.q[hy + 2][hx + 2] = qh;
+            if (hx >= -1 && hx <= TX && hy >= -1 && hy <= TY) L.phi[hy + 1][hx + 1] = phh;
+        }
+        __syncthreads();
+
+        // (c) x faces: fx[fy][f] between tile columns f-1 and f
+        constexpr int NXF = (TX + 1) * TY;
+        for (int j = tid; j < NXF; j += NT) {
+            const int fyi = j / (TX + 1), fxi = j % (TX + 1);
+            L.fx[fyi][fxi] = face_flux<0, M>(L.q[fyi + 2][fxi], L.q[fyi + 2][fxi + 1], L.q[fyi + 2][fxi + 2],
+                                             L.q[fyi + 2][fxi + 3], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1], P);
+        }
+        // y faces: fy[f][fx] between tile rows f-1 and f
+        constexpr int NYF = TX * (TY + 1);
+        for (int j = (tid + 64) % NT; j < NYF; j += NT) {
+            const int fyi = j / TX, fxi = j % TX;
+            L.fy[fyi][fxi] = face_flux<1, M>(L.q[fyi][fxi + 2], L.q[fyi + 1][fxi + 2], L.q[fyi + 2][fxi + 2],
+                                             L.q[fyi + 3][fxi + 2], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1], P);
+        }
+        // own +z face
+        const V4<R> sz1 = limit_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
+        const V4<R> fzp = face_flux_sl<2, M>(qz, sz, qz1, sz1, ph_0, ph_1, P);
+
+        // field stencil from LDS / registers (step.wgsl:118-122)
+        const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
+        const R pym = L.phi[ty][tx + 1], pyp = L.phi[ty + 2][tx + 1];
+        __syncthreads();
+
+        // (d) divergence, sources, RK combination
+        if (valid) {
+            const int64_t i0 = cidx(P, gx, gy, z);
+            R lap_sum = R(0);
+            lap_sum = lap_sum + (pxm + pxp - R(2) * ph_0);
+            const R gxv = R(0.5) * (pxp - pxm) * P.inv_dx;
+            lap_sum = lap_sum + (pym + pyp - R(2) * ph_0);
+            const R gyv = R(0.5) * (pyp - pym) * P.inv_dx;
+            lap_sum = lap_sum + (ph_m + ph_1 - R(2) * ph_0);
+            const R gzv = R(0.5) * (ph_1 - ph_m) * P.inv_dx;
+            const V4<R> fxm = L.fx[ty][tx], fxp = L.fx[ty][tx + 1];
+            const V4<R> fym = L.fy[ty][tx], fyp = L.fy[ty + 1][tx];
+            V4<R> fd{R(0), R(0), R(0), R(0)};
+            fd.x = fd.x + (fxp.x - fxm.x) * P.inv_dx;
+            fd.y = fd.y + (fxp.y - fxm.y) * P.inv_dx;
+            fd.z = fd.z + (fxp.z - fxm.z) * P.inv_dx;
+            fd.w = fd.w + (fxp.w - fxm.w) * P.inv_dx;
+            fd.x = fd.x + (fyp.x - fym.x) * P.inv_dx;
+            fd.y = fd.y + (fyp.y - fym.y) * P.inv_dx;
+            fd.z = fd.z + (fyp.z - fym.z) * P.inv_dx;
+            fd.w = fd.w + (fyp.w - fym.w) * P.inv_dx;
+            fd.x = fd.x + (fzp.x - fzm.x) * P.inv_dx;
+            fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
+            fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
+            fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
+            const V2<R> fin = field_in[i0];
+            const V4<R> uin = fluid_in[i0];
+            R phn, pin;
+            V4<R> un;
+            if (stage == 0) {
+                phn = fin.x; pin = fin.y; un = uin;   // U_in is U^n at stage 0 (same slot)
+            } else {
+                const V2<R> fnn = field_n[i0];
+                phn = fnn.x; pin = fnn.y; un = fluid_n[i0];
+            }
+            R phi_o, pi_o;
+            V4<R> fo;
+            stage_update<M>(ph_0, fin.y, qz, lap_sum, gxv, gyv, gzv, fd, uin, phn, pin, un, a0, a1, dt, P,
+                            phi_o, pi_o, fo);
+            field_out[i0] = V2<R>{phi_o, pi_o};
+            fluid_out[i0] = fo;
+        }
+
+        // (e) rotate the column ring
+        qz = qz1; qz1 = q2; sz = sz1; fzm = fzp;
+        ph_m = ph_0; ph_0 = ph_1; ph_1 = ph_2;
+    }
+}
+
+// Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with
+// tile_x fastest; consecutive blocks land on different XCDs (round-robin
+// dispatch), so the logical tile of block b is remapped such that each XCD
+// works on a contiguous run of tiles and halo re-reads hit its own L2.
+template <int TX, int TY, class M, class R>
+__global__ void __launch_bounds__(TX *TY) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc, int nblocks) {
+    __shared__ FusedLds<TX, TY, R> L;
+    int b = blockIdx.x;
+    if ((nblocks & 7) == 0) b = (b & 7) * (nblocks >> 3) + (b >> 3);
+    const int tx_i = b % ntx;
+    const int ty_i = (b / ntx) % nty;
+    const int zc_i = b / (ntx * nty);
+    const int zb = zc_i * zc;
+    const int ze = min(zb + zc, box.P.nz);
+    int in, out;
+    stage_slot_ids(box.cur, stage, in, out);
+    const int n = box.cur;
+    fused_tile<TX, TY, M, R>(L, box.P, box.field[n], box.fluid[n], box.field[in], box.fluid[in], box.field[out],
+                             box.fluid[out], tx_i * TX, ty_i * TY, zb, ze, stage);
+}
+
+// Multi-box fused stage: one task per block from a task table
+// (box index, x0, y0, zb, ze); each box reads its own `cur` and progress
+// from device memory so a captured graph replays correctly.
+struct Task {
+    int box, x0, y0, zb, ze, pad[3];
+};
+// Device-side progress and nucleation table of one box in a batch: events
+// sorted by the step before which they are stamped.
+struct BoxEvents {
+    const uint64_t *step;   // stamping step of each event
+    const float *bubble;    // 4 floats per event: cx, cy, cz, r0
+    int count;
+    int next;               // first event not yet stamped
+    uint64_t steps_done;    // steps completed by this box
+    uint64_t steps_target;  // the box stops advancing at this count
+    int active;
+    int pad;
+};
+template <int TX, int TY, class M, class R>
+__global__ void __launch_bounds__(TX *TY) k_stage_fused_multi(const Box<R> *__restrict__ boxes,
+                                                              const BoxEvents *__restrict__ events,
+                                                              const Task *__restrict__ tasks, int stage, int ntasks) {
+    __shared__ FusedLds<TX, TY, R> L;
+    int b = blockIdx.x;
+    if ((ntasks & 7) == 0) b = (b & 7) * (ntasks >> 3) + (b >> 3);
+    const Task t = tasks[b];
+    const BoxEvents &ev = events[t.box];
+    if (!ev.active || ev.steps_done >= ev.steps_target) return;  // uniform per block
+    const Box<R> &bx = boxes[t.box];
+    int in, out;
+    stage_slot_ids(bx.cur, stage, in, out);
+    const int n = bx.cur;
+    fused_tile<TX, TY, M, R>(L, bx.P, bx.field[n], bx.fluid[n], bx.field[in], bx.fluid[in], bx.field[out],
+                             bx.fluid[out], t.x0, t.y0, t.zb, t.ze, stage);
+}
+
+// ---------------------------------------------------------------------------
+//  Diagnostics (reduce.wgsl:56-140), accumulated in FP64 with Neumaier
+//  compensation per thread, TwoSum-combined across the wave (shuffles),
+//  the block (LDS) and the fixed-order second pass.  The grid of pass 1 is
+//  fixed (kReduceBlocks), so the summation order depends only on the box.
+// ---------------------------------------------------------------------------
+constexpr int kReduceBlocks = 512;
+constexpr int kReduceThreads = 256;
+
+struct CSum {
+    double s, c;
+};
+__device__ inline void neumaier_add(CSum &a, double x) {
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+__device__ inline CSum csum_merge(CSum a, CSum b) {
+    // TwoSum of the heads, tails added
+    const double s = a.s + b.s;
+    const double bb = s - a.s;
+    const double err = (a.s - (s - bb)) + (b.s - bb);
+    return CSum{s, a.c + b.c + err};
+}
+struct ReducePartial {
+    CSum v[4];
+    double vmax;
+    double nonfinite;
+};
+
+template <class R>
+__global__ void __launch_bounds__(kReduceThreads) k_reduce1(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
+                                                            Params<R> P, ReducePartial *__restrict__ partials) {
+    __shared__ ReducePartial sh[kReduceThreads / 64];
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    CSum acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    double vmax = 0.0;
+    double nonfinite = 0.0;
+    const int64_t stride = (int64_t)kReduceBlocks * kReduceThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kReduceThreads + threadIdx.x; i < n; i += stride) {
+        const int x = (int)(i % P.nx);
+        const int y = (int)((i / P.nx) % P.ny);
+        const int z = (int)(i / ((int64_t)P.nx * P.ny));
+        const V2<R> f = field[i];
+        const R pxm = field[cidx(P, wrapi(x - 1, P.nx), y, z)].x, pxp = field[cidx(P, wrapi(x + 1, P.nx), y, z)].x;
+        const R pym = field[cidx(P, x, wrapi(y - 1, P.ny), z)].x, pyp = field[cidx(P, x, wrapi(y + 1, P.ny), z)].x;
+        const R pzm = field[cidx(P, x, y, wrapi(z - 1, P.nz))].x, pzp = field[cidx(P, x, y, wrapi(z + 1, P.nz))].x;
+        const CellDiag<ExactMath<R>, R> c =
+            cell_contribution<ExactMath<R>, R>(f.x, f.y, pxm, pxp, pym, pyp, pzm, pzp, fluid[i], P);
+        neumaier_add(acc[0], (double)c.e_total);
+        neumaier_add(acc[1], (double)c.broken);
+        neumaier_add(acc[2], (double)c.kinetic);
+        neumaier_add(acc[3], (double)c.e_lab);
+        const double va = (double)c.vabs;
+        vmax = va > vmax ? va : vmax;
+        if (!isfinite(c.e_total) || !isfinite(c.kinetic) || !isfinite(c.vabs)) nonfinite = 1.0;
+    }
+    // wave reduction
+    for (int off = 32; off >= 1; off >>= 1) {
+        for (int k = 0; k < 4; ++k) {
+            CSum o{__shfl_xor(acc[k].s, off), __shfl_xor(acc[k].c, off)};
+            acc[k] = csum_merge(acc[k], o);
+        }
+        const double ov = __shfl_xor(vmax, off);
+        vmax = ov > vmax ? ov : vmax;
+        const double on = __shfl_xor(nonfinite, off);
+        nonfinite = on > nonfinite ? on : nonfinite;
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+        for (int k = 0; k < 4; ++k) sh[wave].v[k] = acc[k];
+        sh[wave].vmax = vmax;
+        sh[wave].nonfinite = nonfinite;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ReducePartial r = sh[0];
+        for (int w = 1; w < kReduceThreads / 64; ++w) {
+            for (int k = 0; k < 4; ++k) r.v[k] = csum_merge(r.v[k], sh[w].v[k]);
+            r.vmax = sh[w].vmax > r.vmax ? sh[w].vmax : r.vmax;
+            r.nonfinite = sh[w].nonfinite > r.nonfinite ? sh[w].nonfinite : r.nonfinite;
+        }
+        partials[blockIdx.x] = r;
+    }
+}
+
+struct ReduceResult {
+    double sums[4];
+    double vmax;
+    double nonfinite;
+};
+
+__global__ void __launch_bounds__(64) k_reduce2(const ReducePartial *__restrict__ partials, ReduceResult *__restrict__ out);
+
+}  // namespace bs

@@ -1,0 +1,52 @@
+// launch.h -- host-callable launchers of the kernels in kernels.h.
+//
+// Defined in kernels_exact.hip (ExactMath, any precision; compiled without
+// FMA contraction) and kernels_fast.hip (FastMathF32; compiled with
+// contraction).  `math` selects between them for f32; f64 is always exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bubblesim.h"
+#include "kernels.h"
+
+namespace bs {
+
+struct TileShape {
+    int tx, ty;
+};
+
+// Tile shapes the fused kernel is instantiated for (x width, y height).
+constexpr int kNumTiles = 4;
+constexpr TileShape kTiles[kNumTiles] = {{32, 16}, {64, 8}, {32, 8}, {16, 16}};
+int tile_index(TileShape t);  // -1 if not instantiated
+
+template <class R> hipError_t launch_init(V2<R> *field, V4<R> *fluid, int64_t n, R e_ref, hipStream_t st);
+template <class R> hipError_t launch_stamp(V2<R> *field, const Params<R> &P, const StampBatch<R> &b, hipStream_t st);
+template <class R>
+hipError_t launch_reduce(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, ReducePartial *partials,
+                         ReduceResult *out, hipStream_t st);
+
+// one fused RK stage over a whole box; zc = planes per z chunk
+template <class R>
+hipError_t launch_stage_fused(int math, const Box<R> &box, int stage, TileShape t, int zc, hipStream_t st);
+// one fused RK stage over a task table spanning many boxes
+template <class R>
+hipError_t launch_stage_fused_multi(int math, const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
+                                    TileShape t, hipStream_t st);
+// primitive pass + stage pass (reference dispatch structure)
+template <class R>
+hipError_t launch_stage_twopass(int math, const Box<R> &box, int stage, V4<R> *prim, hipStream_t st);
+
+// f32 fast-math variants (kernels_fast.hip)
+hipError_t launch_stage_fused_fast(const Box<float> &box, int stage, TileShape t, int zc, hipStream_t st);
+hipError_t launch_stage_fused_multi_fast(const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
+                                         TileShape t, hipStream_t st);
+hipError_t launch_stage_twopass_fast(const Box<float> &box, int stage, V4<float> *prim, hipStream_t st);
+
+// batch bookkeeping kernels (kernels_exact.hip)
+template <class R>
+hipError_t launch_stamp_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
+template <class R> hipError_t launch_advance_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
+
+}  // namespace bs

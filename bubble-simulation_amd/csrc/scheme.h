@@ -1,0 +1,422 @@
+// scheme.h -- the lattice scheme of one SSP-RK3 stage, per cell and per face.
+//
+// __host__ __device__ templates over the real type R (float / double) and an
+// arithmetic policy M (ExactMath / FastMathF32).  Every function names the
+// reference WGSL it restates; the operation order inside each expression is
+// the reference's left-to-right order, so ExactMath evaluations are a fixed,
+// reproducible sequence of IEEE operations.
+//
+// Swap points (docs/EOS.md:10-34): the Eos* and potential functions take the
+// field value phi even where the bag model ignores it, so a phase-dependent
+// equation of state changes no call site.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+namespace bs {
+
+template <class R> struct alignas(4 * sizeof(R)) V4 { R x, y, z, w; };
+template <class R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
+
+// Device-side parameter block: the solver-relevant subset of SimParams
+// (common.wgsl:25-58) in the lattice precision.  Built on the host from the
+// f32 bs_sim_params; for R = double every value is the exact widening of the
+// f32 one, so the FP64 scheme runs the same physics in wider arithmetic.
+template <class R> struct Params {
+    int nx, ny, nz, pad;
+    R dx, inv_dx, dt, eta;
+    R lambda, eps, phi_b, m2;
+    R delta, a_rad, wall_width, p_floor;
+    R e_floor, z_max, e_ref, t_ref;
+};
+
+template <class R> struct Consts;
+template <> struct Consts<float> {
+    static constexpr float tiny = 1e-30f;     // eos.wgsl:109,121, step.wgsl:69
+    static constexpr float den_floor = 1e-12f; // step.wgsl:88, 154
+};
+template <> struct Consts<double> {
+    static constexpr double tiny = 1e-30;
+    static constexpr double den_floor = 1e-12;
+};
+
+// ---------------------------------------------------------------------------
+//  Arithmetic policies
+// ---------------------------------------------------------------------------
+
+// IEEE correctly rounded '/', sqrt; WGSL min/max selection rule
+// ("e2 if e2 < e1 else e1" / "e2 if e1 < e2 else e1"); the literal sign-test
+// minmod of common.wgsl:114-120.
+template <class R> struct ExactMath {
+    static constexpr bool fast = false;
+    __host__ __device__ static inline R div(R a, R b) { return a / b; }
+    __host__ __device__ static inline R sqrt(R x) { return std::sqrt(x); }
+    __host__ __device__ static inline R min(R a, R b) { return b < a ? b : a; }
+    __host__ __device__ static inline R max(R a, R b) { return a < b ? b : a; }
+    __host__ __device__ static inline R abs(R a) { return std::fabs(a); }
+    __host__ __device__ static inline R inv_sqrt(R x) { return R(1) / std::sqrt(x); }
+    __host__ __device__ static inline R sign(R a) { return a > R(0) ? R(1) : (a < R(0) ? R(-1) : R(0)); }
+    // minmod3 (common.wgsl:114-120)
+    __host__ __device__ static inline R minmod3(R a, R b, R c) {
+        const R sa = sign(a);
+        if (sa != sign(b) || sa != sign(c)) return R(0);
+        return sa * min(abs(a), min(abs(b), abs(c)));
+    }
+};
+
+// Hardware reciprocal / square root (1 ulp each, so a*rcp(b) stays inside
+// the 2.5 ulp WGSL allows for '/'), FMA contraction by the compiler, and the
+// minmod written as a median: minmod(a, b) = med3(a, b, 0), so
+// minmod3(c, 2a, 2b) = med3(c, 2 med3(a, b, 0), 0) -- the same value as the
+// sign-test form for every non-NaN input (up to the sign of a zero result).
+struct FastMathF32 {
+    static constexpr bool fast = true;
+    __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static inline float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    __device__ static inline float min(float a, float b) { return fminf(a, b); }
+    __device__ static inline float max(float a, float b) { return fmaxf(a, b); }
+    __device__ static inline float abs(float a) { return fabsf(a); }
+    __device__ static inline float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+    __device__ static inline float minmod3(float a, float b, float c) {
+        return __builtin_amdgcn_fmed3f(a, __builtin_amdgcn_fmed3f(b, c, 0.0f), 0.0f);
+    }
+};
+
+// ---------------------------------------------------------------------------
+//  Small vector helpers (fixed evaluation order)
+// ---------------------------------------------------------------------------
+template <class R> __host__ __device__ inline R dot3(R ax, R ay, R az, R bx, R by, R bz) {
+    return ax * bx + ay * by + az * bz;   // ((ax bx + ay by) + az bz)
+}
+template <class R> __host__ __device__ inline R comp(const V4<R> &v, int i) {
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// ---------------------------------------------------------------------------
+//  Indexing (common.wgsl:80-95)
+// ---------------------------------------------------------------------------
+// Periodic wrap for any offset (the reference's wrap1 covers reach <= 2; a
+// full modulo also serves tiles wider than a small box).
+__host__ __device__ inline int wrapi(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// ---------------------------------------------------------------------------
+//  Potential (potential.wgsl:38-47) -- swap point #1
+// ---------------------------------------------------------------------------
+template <class R> __host__ __device__ inline R potential(R phi, const Params<R> &P) {
+    const R p2 = phi * phi;
+    return R(0.5) * P.m2 * p2 - P.delta * p2 * phi / R(3) + R(0.25) * P.lambda * p2 * p2;
+}
+template <class R> __host__ __device__ inline R dpotential(R phi, const Params<R> &P) {
+    return phi * (P.m2 - P.delta * phi + P.lambda * phi * phi);
+}
+
+// ---------------------------------------------------------------------------
+//  Bag equation of state (eos.wgsl) -- swap point #2
+// ---------------------------------------------------------------------------
+// eos_sound_speed2 (eos.wgsl:50-54): 1/3 for e = 3p.
+template <class R> __host__ __device__ inline R eos_sound_speed2(R /*p*/, R /*phi*/) { return R(1) / R(3); }
+// eos_enthalpy_from_pressure (eos.wgsl:56-59): w = 4p.
+template <class R> __host__ __device__ inline R eos_enthalpy_from_pressure(R p, R /*phi*/) { return R(4) * p; }
+// eos_energy_from_pressure (eos.wgsl:61-63): e = 3p.
+template <class R> __host__ __device__ inline R eos_energy_from_pressure(R p, R /*phi*/) { return R(3) * p; }
+// eos_temperature (eos.wgsl:65-69): T = (3 max(p, p_floor) / a)^(1/4).
+template <class M, class R> __host__ __device__ inline R eos_temperature(R p, R /*phi*/, const Params<R> &P) {
+    const R x = R(3) * M::max(p, P.p_floor) / P.a_rad;
+    return M::sqrt(M::sqrt(x));
+}
+
+// eos_pressure_from_conserved (eos.wgsl:94-98): conjugate root of
+// 3p^2 + 2Ep - (E^2 - Z^2) = 0.
+template <class M, class R>
+__host__ __device__ inline R eos_pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
+    const R disc = M::max(R(4) * e_lab * e_lab - R(3) * z2, R(0));
+    const R p = M::div(e_lab * e_lab - z2, e_lab + M::sqrt(disc));
+    return M::max(p, P.p_floor);
+}
+
+// eos_prim_from_cons with eos_regularise inlined (eos.wgsl:102-123):
+// (E, Z) -> (v, p).
+template <class M, class R>
+__host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params<R> &P) {
+    const R e_lab = M::max(u.x, P.e_floor);
+    R zx = u.y, zy = u.z, zz = u.w;
+    const R zlen = M::sqrt(dot3(zx, zy, zz, zx, zy, zz));
+    const R zcap = P.z_max * e_lab;
+    if (zlen > zcap) {
+        const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
+        zx = zx * f; zy = zy * f; zz = zz * f;
+    }
+    const R p = eos_pressure_from_conserved<M>(e_lab, dot3(zx, zy, zz, zx, zy, zz), phi, P);
+    const R s = e_lab + p;
+    const R sm = M::max(s, Consts<R>::tiny);
+    return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
+}
+
+// eos_cons_from_prim (eos.wgsl:129-135).
+template <class M, class R>
+__host__ __device__ inline V4<R> eos_cons_from_prim(const V4<R> &q, R phi, const Params<R> &P) {
+    const R v2 = M::min(dot3(q.x, q.y, q.z, q.x, q.y, q.z), P.z_max * P.z_max);
+    const R w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
+    return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
+}
+
+// eos_flux (eos.wgsl:140-152) along compile-time axis A.
+template <int A, class R>
+__host__ __device__ inline V4<R> eos_flux(const V4<R> &q, const V4<R> &u) {
+    const R p = q.w;
+    const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
+    V4<R> f{(u.x + p) * vd, u.y * vd, u.z * vd, u.w * vd};
+    if (A == 0) f.y = f.y + p;
+    if (A == 1) f.z = f.z + p;
+    if (A == 2) f.w = f.w + p;
+    return f;
+}
+
+// eos_wave_speeds (eos.wgsl:164-176): relativistic (lambda-, lambda+).
+template <int A, class M, class R>
+__host__ __device__ inline void eos_wave_speeds(const V4<R> &q, R phi, const Params<R> &P, R &lm, R &lp) {
+    const R v2 = M::min(dot3(q.x, q.y, q.z, q.x, q.y, q.z), P.z_max * P.z_max);
+    const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
+    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs = M::sqrt(cs2);
+    const R denom = R(1) - v2 * cs2;
+    const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
+    const R a = M::div(vd * (R(1) - cs2), denom);
+    const R b = M::div(cs * M::sqrt(rad), denom);
+    lm = a - b;
+    lp = a + b;
+}
+
+// sanitise (step.wgsl:63-72): floor p, cap |v| at z_max.
+template <class M, class R>
+__host__ __device__ inline V4<R> sanitise(const V4<R> &q, const Params<R> &P) {
+    const R p = M::max(q.w, P.p_floor);
+    R vx = q.x, vy = q.y, vz = q.z;
+    const R v2 = dot3(vx, vy, vz, vx, vy, vz);
+    const R vmax2 = P.z_max * P.z_max;
+    if (v2 > vmax2) {
+        const R f = M::sqrt(M::div(vmax2, M::max(v2, Consts<R>::tiny)));
+        vx = vx * f; vy = vy * f; vz = vz * f;
+    }
+    return V4<R>{vx, vy, vz, p};
+}
+
+// hll_flux (step.wgsl:77-90) along axis A.
+template <int A, class M, class R>
+__host__ __device__ inline V4<R> hll_flux(const V4<R> &ql, R phil, const V4<R> &qr, R phir,
+                                          const Params<R> &P) {
+    const V4<R> ul = eos_cons_from_prim<M>(ql, phil, P);
+    const V4<R> ur = eos_cons_from_prim<M>(qr, phir, P);
+    const V4<R> fl = eos_flux<A>(ql, ul);
+    const V4<R> fr = eos_flux<A>(qr, ur);
+    R llm, llp, lrm, lrp;
+    eos_wave_speeds<A, M>(ql, phil, P, llm, llp);
+    eos_wave_speeds<A, M>(qr, phir, P, lrm, lrp);
+    const R s_min = M::min(M::min(llm, lrm), R(0));
+    const R s_max = M::max(M::max(llp, lrp), R(0));
+    const R den = M::max(s_max - s_min, Consts<R>::den_floor);
+    const R smm = s_min * s_max;
+    return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
+                 M::div(s_max * fl.y - s_min * fr.y + smm * (ur.y - ul.y), den),
+                 M::div(s_max * fl.z - s_min * fr.z + smm * (ur.z - ul.z), den),
+                 M::div(s_max * fl.w - s_min * fr.w + smm * (ur.w - ul.w), den)};
+}
+
+// limit_slope / limit_slope4 (common.wgsl:125-136): monotonised central.
+template <class M, class R> __host__ __device__ inline R limit_slope(R dm, R dp) {
+    return M::minmod3(R(0.5) * (dm + dp), R(2) * dm, R(2) * dp);
+}
+template <class M, class R>
+__host__ __device__ inline V4<R> limit_slope4(const V4<R> &dm, const V4<R> &dp) {
+    return V4<R>{limit_slope<M>(dm.x, dp.x), limit_slope<M>(dm.y, dp.y),
+                 limit_slope<M>(dm.z, dp.z), limit_slope<M>(dm.w, dp.w)};
+}
+template <class R> __host__ __device__ inline V4<R> sub4(const V4<R> &a, const V4<R> &b) {
+    return V4<R>{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
+}
+// q + 0.5 s  and  q - 0.5 s  (step.wgsl:136-143)
+template <class R> __host__ __device__ inline V4<R> half_plus(const V4<R> &q, const V4<R> &s) {
+    return V4<R>{q.x + R(0.5) * s.x, q.y + R(0.5) * s.y, q.z + R(0.5) * s.z, q.w + R(0.5) * s.w};
+}
+template <class R> __host__ __device__ inline V4<R> half_minus(const V4<R> &q, const V4<R> &s) {
+    return V4<R>{q.x - R(0.5) * s.x, q.y - R(0.5) * s.y, q.z - R(0.5) * s.z, q.w - R(0.5) * s.w};
+}
+
+// The flux through the face between cells L and R along axis A, given the
+// sanitised primitives of the four cells L-1, L, R, R+1 (step.wgsl:125-145:
+// this is f_plus of cell L and f_minus of cell R; the reference evaluates it
+// once from each side, with identical operands).
+template <int A, class M, class R>
+__host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, const V4<R> &qr,
+                                           const V4<R> &qrp, R phil, R phir, const Params<R> &P) {
+    const V4<R> sl = limit_slope4<M>(sub4(ql, qlm), sub4(qr, ql));
+    const V4<R> sr = limit_slope4<M>(sub4(qr, ql), sub4(qrp, qr));
+    return hll_flux<A, M>(sanitise<M>(half_plus(ql, sl), P), phil,
+                          sanitise<M>(half_minus(qr, sr), P), phir, P);
+}
+// Same, with the two slopes already known.
+template <int A, class M, class R>
+__host__ __device__ inline V4<R> face_flux_sl(const V4<R> &ql, const V4<R> &sl, const V4<R> &qr,
+                                              const V4<R> &sr, R phil, R phir, const Params<R> &P) {
+    return hll_flux<A, M>(sanitise<M>(half_plus(ql, sl), P), phil,
+                          sanitise<M>(half_minus(qr, sr), P), phir, P);
+}
+
+// ---------------------------------------------------------------------------
+//  Per-cell source terms and the RK stage combination (step.wgsl:150-173)
+// ---------------------------------------------------------------------------
+// Given phi0, pi0, the sanitised primitive q0, the Laplacian sum, the
+// gradient, the flux divergence (already scaled by inv_dx), and U_in / U^n of
+// the cell, produce U_out.
+template <class M, class R>
+__host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R lap_sum, R gx, R gy, R gz,
+                                             const V4<R> &flux_div, const V4<R> &fin,
+                                             R phin, R pin, const V4<R> &fn,
+                                             R a0, R a1, R dt, const Params<R> &P,
+                                             R &phi_out, R &pi_out, V4<R> &f_out) {
+    const R laplacian = lap_sum * P.inv_dx * P.inv_dx;
+    const R gam = M::inv_sqrt(M::max(R(1) - dot3(q0.x, q0.y, q0.z, q0.x, q0.y, q0.z), Consts<R>::den_floor));
+    const R dphi_proper = gam * (pi0 + dot3(q0.x, q0.y, q0.z, gx, gy, gz));
+    const R drag = P.eta * dphi_proper;
+    const R d_phi = pi0;
+    const R d_pi = laplacian - dpotential(phi0, P) - drag;
+    V4<R> d{-flux_div.x, -flux_div.y, -flux_div.z, -flux_div.w};
+    d.x = d.x + drag * pi0;
+    d.y = d.y - drag * gx;
+    d.z = d.z - drag * gy;
+    d.w = d.w - drag * gz;
+    const R up_phi = phi0 + dt * d_phi;
+    const R up_pi = pi0 + dt * d_pi;
+    const V4<R> up{fin.x + dt * d.x, fin.y + dt * d.y, fin.z + dt * d.z, fin.w + dt * d.w};
+    phi_out = a0 * phin + a1 * up_phi;
+    pi_out = a0 * pin + a1 * up_pi;
+    f_out = V4<R>{a0 * fn.x + a1 * up.x, a0 * fn.y + a1 * up.y, a0 * fn.z + a1 * up.z, a0 * fn.w + a1 * up.w};
+}
+
+// ---------------------------------------------------------------------------
+//  Nucleation profile (nucleate.wgsl:52-85)
+// ---------------------------------------------------------------------------
+// tanh from expm1, both written out here so that every build (GPU f32/f64,
+// host) evaluates the identical operation sequence:
+//   expm1(a) for a >= 0: a = k ln2 + r, |r| <= ln2/2 (Cody-Waite split of
+//   ln2), expm1(r) by its Taylor series (degree 9 for float, 16 for double),
+//   expm1(a) = 2^k (expm1(r) + 1) - 1;
+//   tanh(x) = sign(x) * e / (e + 2) with e = expm1(2|x|).
+template <class R> struct TanhK;
+template <> struct TanhK<float> {
+    static constexpr int terms = 9;
+    static constexpr float ln2_hi = 0.693145751953125f;        // 12-bit head of ln 2
+    static constexpr float ln2_lo = 1.428606765330187045e-06f; // ln 2 - ln2_hi
+    static constexpr float inv_ln2 = 1.44269504088896341f;
+    static constexpr float big = 9.0f;                          // tanh(x) == 1 in f32 beyond this
+};
+template <> struct TanhK<double> {
+    static constexpr int terms = 16;
+    static constexpr double ln2_hi = 6.93147180369123816490e-01;
+    static constexpr double ln2_lo = 1.90821492927058770002e-10;
+    static constexpr double inv_ln2 = 1.44269504088896338700e+00;
+    static constexpr double big = 19.5;
+};
+
+template <class R> __host__ __device__ inline R ldexp_int(R x, int k) {
+    // multiply by 2^k with k in [-60, 60], as exact power-of-two factors
+    R f = R(1);
+    R two = k >= 0 ? R(2) : R(0.5);
+    int n = k >= 0 ? k : -k;
+    while (n > 0) { f = f * two; --n; }
+    return x * f;
+}
+
+template <class R> __host__ __device__ inline R bs_expm1_pos(R a) {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+    // a in [0, 2*big]
+    const R kf = std::floor(a * TanhK<R>::inv_ln2 + R(0.5));
+    const int k = (int)kf;
+    const R r = (a - kf * TanhK<R>::ln2_hi) - kf * TanhK<R>::ln2_lo;
+    // Horner for r + r^2/2! + ... + r^n/n!
+    R t = R(1);
+    for (int i = TanhK<R>::terms; i >= 2; --i) t = R(1) + t * r / R(i);
+    const R em1r = r * t;
+    if (k == 0) return em1r;
+    return ldexp_int(em1r + R(1), k) - R(1);
+}
+
+template <class R> __host__ __device__ inline R bs_tanh(R x) {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+    const R ax = std::fabs(x);
+    R t;
+    if (ax >= TanhK<R>::big) {
+        t = R(1);
+    } else {
+        const R e = bs_expm1_pos(R(2) * ax);
+        t = e / (e + R(2));
+    }
+    return x < R(0) ? -t : t;
+}
+
+// min_image (nucleate.wgsl:52-57)
+template <class R> __host__ __device__ inline R min_image(R d, R n) {
+    R r = d;
+    if (r > R(0.5) * n) r = r - n;
+    if (r < R(-0.5) * n) r = r + n;
+    return r;
+}
+
+// One bubble's profile value at cell (x, y, z) (nucleate.wgsl:271-281).
+template <class R>
+__host__ __device__ inline R bubble_profile(int x, int y, int z, R cx, R cy, R cz, R r0, const Params<R> &P) {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+    const R dx_ = min_image(R(x) - cx, R(P.nx));
+    const R dy_ = min_image(R(y) - cy, R(P.ny));
+    const R dz_ = min_image(R(z) - cz, R(P.nz));
+    const R r = std::sqrt(dot3(dx_, dy_, dz_, dx_, dy_, dz_)) * P.dx;
+    return R(0.5) * P.phi_b * (R(1) - bs_tanh((r - r0 * P.dx) / P.wall_width));
+}
+
+// ---------------------------------------------------------------------------
+//  Diagnostics contribution of one cell (reduce.wgsl:56-79, 96)
+// ---------------------------------------------------------------------------
+template <class M, class R> struct CellDiag {
+    R e_total, broken, kinetic, e_lab, vabs;
+};
+template <class M, class R>
+__host__ __device__ inline CellDiag<M, R> cell_contribution(R phi, R pi, R phixm, R phixp, R phiym, R phiyp,
+                                                            R phizm, R phizp, const V4<R> &u,
+                                                            const Params<R> &P) {
+    R grad2 = R(0);
+    {
+        const R d = R(0.5) * (phixp - phixm) * P.inv_dx;
+        grad2 = grad2 + d * d;
+    }
+    {
+        const R d = R(0.5) * (phiyp - phiym) * P.inv_dx;
+        grad2 = grad2 + d * d;
+    }
+    {
+        const R d = R(0.5) * (phizp - phizm) * P.inv_dx;
+        grad2 = grad2 + d * d;
+    }
+    const R e_lab = M::max(u.x, P.e_floor);
+    const V4<R> q = eos_prim_from_cons<M>(u, phi, P);
+    const R v2 = dot3(q.x, q.y, q.z, q.x, q.y, q.z);
+    CellDiag<M, R> c;
+    c.e_total = R(0.5) * pi * pi + R(0.5) * grad2 + potential(phi, P) + e_lab;
+    c.broken = phi > R(0.5) * P.phi_b ? R(1) : R(0);
+    c.kinetic = (e_lab + q.w) * v2;
+    c.e_lab = e_lab;
+    c.vabs = M::sqrt(v2);
+    return c;
+}
+
+}  // namespace bs

@@ -1,0 +1,432 @@
+/*
+ * bubblesim.h -- C ABI of the MI355X-native lattice solver for a first-order
+ * cosmological phase transition (scalar field phi/pi coupled by friction eta to
+ * a relativistic bag-EOS fluid, periodic cubic lattice, SSP-RK3).
+ *
+ * The reference (AgenttiX/bubble-simulation) has no FFI: its boundary is the
+ * Rust `Simulation` struct API plus a handful of #[repr(C)] byte layouts that
+ * mirror WGSL uniforms.  Every declaration below names the reference item it
+ * replaces (file:line, relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every entry point that can fail returns an int status (BS_OK == 0) and
+ *     leaves a human-readable message in a thread-local string read with
+ *     bs_last_error().  No entry point aborts the process.
+ *   - Handles (bs_sim, bs_batch, bs_server) are opaque and owned by the
+ *     library; destroy them with the matching *_destroy call.
+ *   - The library owns all device memory.  Lattice state only crosses the
+ *     boundary through the explicit download/upload debug calls.
+ *   - One handle per host thread: calls on one bs_sim/bs_batch are not
+ *     thread-safe.  bs_server_submit/poll/wait are thread-safe.
+ *   - State layout (download/upload): two arrays per lattice, x fastest,
+ *     linear index i = (z*ny + y)*nx + x  (common.wgsl:19-23):
+ *       field[i] = (phi, pi)                 2 reals
+ *       fluid[i] = (E, Zx, Zy, Zz)           4 reals
+ *     where a real is float for BS_PRECISION_F32 and double for BS_PRECISION_F64.
+ */
+#ifndef BUBBLESIM_H
+#define BUBBLESIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/*  Status codes (the reference uses anyhow::Result; sim.rs:764-781,        */
+/*  physics.rs:191-217, headless.rs:63-67)                                  */
+/* ------------------------------------------------------------------------ */
+enum {
+    BS_OK = 0,
+    BS_ERR_INVALID = 1,   /* invalid parameters (Model::validate, Config::validate) */
+    BS_ERR_OOM = 2,       /* device memory exhausted or over budget              */
+    BS_ERR_DEVICE = 3,    /* HIP runtime error                                   */
+    BS_ERR_NONFINITE = 4, /* non-finite diagnostics (headless.rs:63-67)          */
+    BS_ERR_NOT_READY = 5, /* asynchronous result not available yet              */
+    BS_ERR_NOT_FOUND = 6  /* unknown job id / box index                          */
+};
+
+/* Lattice precision of a simulation.  F32 is the reference's arithmetic
+ * (physics.rs:21 `Scalar = f32`); F64 is the same scheme in double precision,
+ * used as the accuracy oracle (docs/PRECISION.md:154-177). */
+enum { BS_PRECISION_F32 = 0, BS_PRECISION_F64 = 1 };
+
+/* Arithmetic policy of the stage kernels.
+ *   BS_MATH_FAST : hardware reciprocal/square root (<= 2 ulp, inside WGSL's
+ *                  2.5 ulp bound for `/` and `sqrt`) and FMA contraction.
+ *   BS_MATH_EXACT: IEEE correctly rounded division/square root, no FMA
+ *                  contraction, reference operation order; bit-identical to a
+ *                  sequential CPU evaluation of the same operation sequence.
+ * F64 lattices always use EXACT. */
+enum { BS_MATH_FAST = 0, BS_MATH_EXACT = 1 };
+
+/* Stage kernel implementation.
+ *   BS_KERNEL_FUSED  : one fused launch per RK stage (primitive recovery,
+ *                      MUSCL/HLL fluxes computed once per face, friction,
+ *                      field update and stage combination) with an LDS plane
+ *                      ring marching along z.
+ *   BS_KERNEL_TWO_PASS: primitive pass + stage pass per RK stage, the
+ *                      reference's dispatch structure (sim.rs:881-916). */
+enum { BS_KERNEL_FUSED = 0, BS_KERNEL_TWO_PASS = 1 };
+
+/* ------------------------------------------------------------------------ */
+/*  Physics model (physics.rs:23-83)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* SeedSize (physics.rs:29-42) */
+enum { BS_SEED_CRITICAL = 0, BS_SEED_CELLS = 1 };
+
+/* NucleationMode (physics.rs:224-234).  UNIFORM_STEPS is an extension: times
+ * are whole steps drawn uniformly in [0, uniform_max_step] from a generator
+ * seeded by `time_seed`, separate from the position stream, so positions stay
+ * identical to the reference's for the same `seed`. */
+enum { BS_NUC_SIMULTANEOUS = 0, BS_NUC_EXPONENTIAL = 1, BS_NUC_UNIFORM_STEPS = 2 };
+
+/* Model (physics.rs:44-68).  seed_kind/seed_value encode SeedSize. */
+typedef struct bs_model {
+    float lambda;
+    float eps_ratio;
+    float alpha;
+    float eta;
+    float phi_b;
+    float dx;
+    float cfl;
+    int32_t seed_kind;   /* BS_SEED_CRITICAL: factor of R_c; BS_SEED_CELLS: cells */
+    float seed_value;
+} bs_model;
+
+/* Derived quantities (physics.rs:90-189), all evaluated in f32 in the
+ * reference's operation order. */
+typedef struct bs_derived {
+    float eps;                     /* physics.rs:90-92   */
+    float m2;                      /* physics.rs:95-97   */
+    float delta;                   /* physics.rs:100-102 */
+    float wall_width;              /* physics.rs:109-111 */
+    float surface_tension;         /* physics.rs:114-116 */
+    float critical_radius;         /* physics.rs:120-122 */
+    float e_ref;                   /* physics.rs:125-127 */
+    float t_ref;                   /* physics.rs:129-131 */
+    float a_rad;                   /* physics.rs:134-136 */
+    float dt;                      /* physics.rs:138-140 */
+    float seed_radius;             /* physics.rs:154-160 */
+    float critical_to_wall_ratio;  /* physics.rs:173-175 */
+    float estimated_wall_velocity; /* physics.rs:180-184 */
+    float sound_speed;             /* physics.rs:187-189 */
+} bs_derived;
+
+/* BubbleEvent (physics.rs:236-241) */
+typedef struct bs_event {
+    float time;
+    float pos[3];
+    float radius;
+} bs_event;
+
+/* ------------------------------------------------------------------------ */
+/*  Byte layouts mirrored from the WGSL uniforms (sim.rs:66-131)            */
+/* ------------------------------------------------------------------------ */
+
+/* SimParams (common.wgsl:25-58, SimParamsGpu sim.rs:66-98): 96 bytes. */
+typedef struct bs_sim_params {
+    uint32_t nx, ny, nz, pad_u0;
+    float dx, inv_dx, dt, eta;
+    float lambda, eps, phi_b, m2;
+    float delta, a_rad, wall_width, p_floor;
+    float e_floor, z_max, e_ref, t_ref;
+    float time, vis_gain, pad0, pad1;
+} bs_sim_params;
+
+/* StageParams (common.wgsl:63-68, StageParamsGpu sim.rs:100-107): 16 bytes. */
+typedef struct bs_stage_params {
+    float a0, a1, dt_stage, pad;
+} bs_stage_params;
+
+/* Bubble (nucleate.wgsl:29-35, BubbleGpu sim.rs:109-116): 16 bytes. */
+typedef struct bs_bubble {
+    float cx, cy, cz, r0;
+} bs_bubble;
+
+/* NucleationBatch (nucleate.wgsl:40-45, sim.rs:118-123): 16 bytes. */
+typedef struct bs_nucleation_batch {
+    uint32_t count, pad[3];
+} bs_nucleation_batch;
+
+/* Diagnostics readback block (reduce.wgsl:32-38, DiagnosticsGpu
+ * sim.rs:125-131): 32 bytes.  Filled by bs_sim_diagnostics_raw with the
+ * FP64-accumulated sums rounded to f32. */
+typedef struct bs_diagnostics_gpu {
+    float sums[4];
+    uint32_t max_v_bits;
+    uint32_t pad[3];
+} bs_diagnostics_gpu;
+
+/* Host diagnostics (Diagnostics, sim.rs:141-167).  Means are the FP64
+ * compensated sums divided by the cell count in FP64; the reference sums in
+ * f32 and divides in f32 (reduce.wgsl:81-140, sim.rs:991-1010). */
+typedef struct bs_diagnostics {
+    double mean_energy;        /* <1/2 pi^2 + 1/2 |grad phi|^2 + V + max(E,e_floor)> */
+    double broken_fraction;    /* <[phi > phi_b/2]>                                   */
+    double mean_kinetic;       /* <(max(E,e_floor) + p) v^2>                          */
+    double mean_fluid_energy;  /* <max(E,e_floor)>                                    */
+    double max_velocity;       /* max |v|                                             */
+    double kinetic_fraction;   /* Diagnostics::kinetic_fraction, sim.rs:160-166       */
+    uint64_t cells;
+    uint32_t nonfinite;        /* non-zero if any summand was NaN/Inf                 */
+    uint32_t pad;
+} bs_diagnostics;
+
+/* Everything Simulation::new needs (SimulationSpec sim.rs:246-255, built by
+ * Config::sim_spec config.rs:138-149), plus this library's extensions. */
+typedef struct bs_spec {
+    bs_model model;
+    uint32_t grid[3];
+    int32_t nucleation;          /* BS_NUC_*                                   */
+    uint32_t bubbles;            /* config.rs:65-67                            */
+    float nucleation_duration;   /* config.rs:73-75                            */
+    uint64_t seed;               /* config.rs:77-79                            */
+    /* extensions */
+    int32_t precision;           /* BS_PRECISION_*                             */
+    int32_t math;                /* BS_MATH_*                                  */
+    int32_t kernel;              /* BS_KERNEL_*                                */
+    int32_t bubble_at_centre;    /* 1: single bubble at (nx/2, ny/2, nz/2)     */
+    uint32_t uniform_max_step;   /* BS_NUC_UNIFORM_STEPS: times in [0, this]   */
+    uint32_t pad0;
+    uint64_t time_seed;          /* BS_NUC_UNIFORM_STEPS time generator seed   */
+} bs_spec;
+
+/* ------------------------------------------------------------------------ */
+/*  Library / error surface                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *bs_last_error(void);
+/* Library version string. */
+const char *bs_version(void);
+/* sizeof of each ABI struct, so bindings can check their mirrors
+ * (parameter_blocks_have_the_expected_size, sim.rs:1070-1078).
+ * which: 0 sim_params, 1 stage_params, 2 bubble, 3 nucleation_batch,
+ * 4 diagnostics_gpu, 5 diagnostics, 6 spec, 7 model, 8 event, 9 derived. */
+size_t bs_struct_size(int which);
+
+/* ------------------------------------------------------------------------ */
+/*  Host physics (physics.rs, config.rs, sim.rs host helpers)               */
+/* ------------------------------------------------------------------------ */
+
+/* Model::default (physics.rs:70-83). */
+void bs_model_default(bs_model *out);
+/* Model::validate (physics.rs:191-217).  BS_OK or BS_ERR_INVALID. */
+int bs_model_validate(const bs_model *m);
+/* Model derivations (physics.rs:90-189). */
+int bs_model_derive(const bs_model *m, bs_derived *out);
+/* make_schedule (physics.rs:248-296): Pcg64Mcg positions with 2.5 R0
+ * rejection (<= 64 attempts), simultaneous/exponential times, stable sort by
+ * time.  `out` holds `count` events.  Bit-exact with the reference for
+ * BS_NUC_SIMULTANEOUS and BS_NUC_EXPONENTIAL.  For BS_NUC_UNIFORM_STEPS,
+ * `duration` is unused and times are k*dt for k drawn in [0, uniform_max_step]
+ * from `time_seed`; `dt` must then be the model's time step. */
+int bs_make_schedule(int32_t mode, uint32_t count, float radius, const uint32_t grid[3],
+                     float duration, uint64_t seed, uint32_t uniform_max_step,
+                     uint64_t time_seed, float dt, bs_event *out);
+/* Raw Pcg64Mcg stream (rand_pcg 0.9 Mcg128Xsl64, seeded by rand_core 0.9
+ * SeedableRng::seed_from_u64) -- the generator make_schedule draws from
+ * (physics.rs:256, 278-280).  Writes n next_u64 outputs. */
+void bs_pcg64mcg_u64(uint64_t seed, uint32_t n, uint64_t *out);
+/* rand 0.9 `random::<f32>()` on the same stream: n floats in [0, 1). */
+void bs_pcg64mcg_f32(uint64_t seed, uint32_t n, float *out);
+/* Mcg128Xsl64::new(state) then n next_u64 (state given as lo/hi halves). */
+void bs_mcg128_from_state(uint64_t state_lo, uint64_t state_hi, uint32_t n, uint64_t *out);
+
+/* gpu_params (sim.rs:613-644): the 96-byte parameter block for a model and
+ * grid at time `time`. */
+int bs_sim_params_fill(const bs_model *m, const uint32_t grid[3], float time,
+                       bs_sim_params *out);
+/* RK3 (sim.rs:135) as (a0, a1) x 3 -> out[6]; upload_stage_params
+ * (sim.rs:650-656) uses dt_stage = dt for every stage. */
+void bs_rk3_coefficients(float out[6]);
+/* stage_slots (sim.rs:1046-1051): out[2*s] = input slot, out[2*s+1] = output
+ * slot of stage s, for U^n in slot `cur`. */
+void bs_stage_slots(uint32_t cur, uint32_t out[6]);
+/* Step index at which each event is stamped: the first step s whose f32
+ * accumulated time (sum of s dt's, sim.rs:913-915) is >= event time, as
+ * record_nucleation pops events with time <= self.time before each step
+ * (sim.rs:843-879, headless.rs:126-131). */
+int bs_event_steps(const bs_event *events, uint32_t n, float dt, uint64_t *steps_out);
+
+/* Config::validate (config.rs:151-178) applied to a spec. */
+int bs_spec_validate(const bs_spec *spec);
+/* The spec Config::parse_from(["bubble-simulation"]) produces (config.rs
+ * defaults), with precision F32, fast math and the fused kernel. */
+void bs_spec_default(bs_spec *out);
+/* Parse a --grid value: "192" or "256,128,64" (parse_grid config.rs:194-203). */
+int bs_parse_grid(const char *s, uint32_t grid_out[3]);
+
+/* Device bytes per lattice cell held by a simulation of this precision:
+ * three RK slots of (field 2 + fluid 4 reals), plus the primitive cache of
+ * the two-pass kernel (4 reals) when `kernel` is BS_KERNEL_TWO_PASS.  The
+ * reference's figure is 96 B/cell (Simulation::lattice_bytes sim.rs:705-716). */
+uint64_t bs_bytes_per_cell(int32_t precision, int32_t kernel);
+/* cells * bs_bytes_per_cell (lattice_bytes sim.rs:712-716). */
+uint64_t bs_lattice_bytes(const uint32_t grid[3], int32_t precision, int32_t kernel);
+/* max_cubic_size (sim.rs:718-734) with a memory budget only (HIP has no
+ * storage-binding ceiling).  side is clamped to [8, 2048]. */
+int bs_max_cubic_size(uint64_t budget_bytes, int32_t precision, int32_t kernel,
+                      uint32_t *side_out, int32_t *limited_by_memory_out);
+/* lattice_budget (vram.rs:58,67-69): (available + current) * 0.85. */
+uint64_t bs_lattice_budget(uint64_t available, uint64_t current_lattice);
+
+/* ------------------------------------------------------------------------ */
+/*  Device                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Number of visible HIP devices (0 if none). */
+int bs_device_count(void);
+/* Free/total device memory of the current device (hipMemGetInfo). */
+int bs_device_memory(uint64_t *free_out, uint64_t *total_out);
+/* Device name into buf. */
+int bs_device_name(char *buf, size_t len);
+
+/* ------------------------------------------------------------------------ */
+/*  Simulation (Simulation, sim.rs:180-1026)                                */
+/* ------------------------------------------------------------------------ */
+typedef struct bs_sim bs_sim;
+
+/* Simulation::new (sim.rs:257-607): validate, allocate, build the schedule
+ * (make_schedule), init the lattice (init.wgsl) at time 0. */
+int bs_sim_create(const bs_spec *spec, bs_sim **out);
+void bs_sim_destroy(bs_sim *sim);
+/* Simulation::reset (sim.rs:790-821).  schedule == NULL keeps the current
+ * schedule; otherwise it is replaced by the n given events. */
+int bs_sim_reset(bs_sim *sim, const bs_event *schedule, uint32_t n);
+/* Simulation::set_model (sim.rs:663-670).  An invalid model is ignored and
+ * BS_ERR_INVALID returned; the simulation is untouched. */
+int bs_sim_set_model(bs_sim *sim, const bs_model *m);
+/* Simulation::record_steps (sim.rs:829-841): stamp due events once, then n
+ * steps.  Asynchronous on the simulation's stream. */
+int bs_sim_record_steps(bs_sim *sim, uint32_t n);
+/* headless::advance's stepping loop (headless.rs:126-131): n x
+ * record_steps(1), i.e. due events are stamped before every step. */
+int bs_sim_advance(bs_sim *sim, uint64_t n);
+/* record_diagnostics + after_submit + poll_diagnostics (sim.rs:951-1010),
+ * blocking.  Primitives are recomputed from the current state inside the
+ * reduction, so no separate refresh is needed (headless.rs:135-137). */
+int bs_sim_diagnostics(bs_sim *sim, bs_diagnostics *out);
+/* The same reduction as the reference's 32-byte block. */
+int bs_sim_diagnostics_raw(bs_sim *sim, bs_diagnostics_gpu *out);
+/* Non-blocking form: request enqueues the reduction, poll returns
+ * BS_ERR_NOT_READY until it has completed (after_submit/poll_diagnostics). */
+int bs_sim_request_diagnostics(bs_sim *sim);
+int bs_sim_poll_diagnostics(bs_sim *sim, bs_diagnostics *out);
+/* Block until all queued work on the simulation's stream has finished. */
+int bs_sim_synchronize(bs_sim *sim);
+/* Simulation::resize (sim.rs:736-762): validates first; on failure the
+ * simulation is unchanged.  Re-initialises the lattice. */
+int bs_sim_resize(bs_sim *sim, const uint32_t grid[3]);
+/* Simulation::nucleate_now (sim.rs:1012-1025): queue a bubble at `pos`
+ * (lattice cells) stamped before the next step. */
+int bs_sim_nucleate_now(bs_sim *sim, const float pos[3]);
+/* Public fields / getters (sim.rs:180-242, 686-703). */
+uint64_t bs_sim_cell_count(const bs_sim *sim);
+uint32_t bs_sim_remaining_bubbles(const bs_sim *sim);
+float bs_sim_time(const bs_sim *sim);
+uint64_t bs_sim_steps(const bs_sim *sim);
+int32_t bs_sim_nucleated_this_frame(const bs_sim *sim);
+uint64_t bs_sim_device_memory_bytes(const bs_sim *sim);
+int bs_sim_grid(const bs_sim *sim, uint32_t grid_out[3]);
+int bs_sim_model(const bs_sim *sim, bs_model *out);
+/* Copy of the schedule (remaining + already stamped), n_out = total. */
+int bs_sim_schedule(const bs_sim *sim, bs_event *out, uint32_t cap, uint32_t *n_out);
+/* Debug dump / load of U^n (slot `cur`), layout in the header comment. */
+int bs_sim_download(bs_sim *sim, void *field, void *fluid);
+int bs_sim_upload(bs_sim *sim, const void *field, const void *fluid);
+/* Run n steps (advance semantics) and report the summed duration of the
+ * stage-kernel launches, measured with HIP events on the simulation's stream
+ * around each launch, and the wall time of the whole call. */
+int bs_sim_advance_timed(bs_sim *sim, uint64_t n, double *stage_kernel_ms, double *wall_ms);
+
+/* ------------------------------------------------------------------------ */
+/*  Batched sweep: many independent boxes advanced together (no reference   */
+/*  counterpart; each box follows Simulation semantics exactly)            */
+/* ------------------------------------------------------------------------ */
+typedef struct bs_batch bs_batch;
+
+/* All specs must share precision and math.  Boxes may differ in grid, model
+ * and schedule. */
+int bs_batch_create(const bs_spec *specs, uint32_t n, bs_batch **out);
+void bs_batch_destroy(bs_batch *b);
+/* Advance every box by n steps (advance semantics per box).  use_graph != 0
+ * captures steps into a hipGraph and replays it. */
+int bs_batch_advance(bs_batch *b, uint64_t n, int32_t use_graph);
+int bs_batch_synchronize(bs_batch *b);
+uint32_t bs_batch_size(const bs_batch *b);
+uint64_t bs_batch_total_cells(const bs_batch *b);
+int bs_batch_diagnostics(bs_batch *b, uint32_t box, bs_diagnostics *out);
+int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid);
+int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out);
+/* Tile shape chosen by the autotuner for this batch (x, y). */
+int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty);
+
+/* ------------------------------------------------------------------------ */
+/*  Serving: continuous batching of sweep jobs under an HBM budget          */
+/* ------------------------------------------------------------------------ */
+typedef struct bs_server bs_server;
+
+typedef struct bs_server_config {
+    uint64_t hbm_budget_bytes;   /* 0: use (total device memory - reserve)     */
+    uint64_t reserve_bytes;      /* kept free                                  */
+    uint32_t max_active;         /* 0: unlimited                               */
+    uint32_t chunk_steps;        /* steps per scheduling tick (>= 1)           */
+    int32_t precision;           /* BS_PRECISION_* for every job               */
+    int32_t math;                /* BS_MATH_*                                  */
+} bs_server_config;
+
+typedef struct bs_job {
+    bs_spec spec;
+    uint64_t steps;              /* steps to advance                            */
+    int32_t keep_state;          /* 1: keep the final state for bs_server_fetch */
+    uint32_t pad;
+} bs_job;
+
+typedef struct bs_job_result {
+    uint64_t job_id;
+    uint64_t steps;
+    bs_diagnostics final_diag;
+    double submit_s;             /* seconds since server start                 */
+    double admit_s;
+    double finish_s;
+    double latency_s;            /* finish - submit                            */
+    uint64_t state_checksum;     /* FNV-1a of the final field+fluid bytes      */
+    int32_t status;              /* BS_OK or an error code                      */
+    uint32_t pad;
+} bs_job_result;
+
+typedef struct bs_server_stats {
+    uint64_t submitted, admitted, completed, rejected;
+    uint64_t site_updates;       /* sum over completed jobs of cells * steps   */
+    uint64_t ticks;
+    uint64_t bytes_in_use;
+    uint64_t budget_bytes;
+    uint32_t active, queued;
+    double busy_s;               /* wall time spent advancing                  */
+    double elapsed_s;            /* since start                                */
+} bs_server_stats;
+
+int bs_server_create(const bs_server_config *cfg, bs_server **out);
+/* Thread-safe.  Rejects jobs that can never fit the budget (BS_ERR_OOM) and
+ * invalid specs (BS_ERR_INVALID). */
+int bs_server_submit(bs_server *s, const bs_job *job, uint64_t *job_id_out);
+/* BS_OK with the result once the job has completed, BS_ERR_NOT_READY before. */
+int bs_server_poll(bs_server *s, uint64_t job_id, bs_job_result *out);
+/* Block until the job completes or timeout_s elapses (BS_ERR_NOT_READY). */
+int bs_server_wait(bs_server *s, uint64_t job_id, double timeout_s, bs_job_result *out);
+/* Final state of a job submitted with keep_state (layout as bs_sim_download). */
+int bs_server_fetch_state(bs_server *s, uint64_t job_id, void *field, void *fluid);
+int bs_server_stats_get(bs_server *s, bs_server_stats *out);
+/* Stop accepting work, finish nothing further, free device memory. */
+void bs_server_destroy(bs_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BUBBLESIM_H */

@@ -1,0 +1,638 @@
+// oracle.cpp -- CPU restatement of the reference lattice solver.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke
+// check and the CPU-baseline leg of the benchmark load it; the library never
+// does.  It restates, operation by operation and in the reference's
+// left-to-right expression order, the WGSL solver of AgenttiX/bubble-simulation:
+//
+//   shaders/common.wgsl     wrap1, cell_index, minmod3, limit_slope(4)
+//   shaders/potential.wgsl  potential, dpotential
+//   shaders/eos.wgsl        pressure_from_conserved, regularise, prim_from_cons,
+//                           cons_from_prim, flux, wave_speeds
+//   shaders/primitives.wgsl primitive pass
+//   shaders/step.wgsl       sanitise, hll_flux, the stage (faces from both sides)
+//   shaders/init.wgsl       initial condition
+//   shaders/nucleate.wgsl   min_image, tanh profile, max-merge
+//   shaders/reduce.wgsl     cell_contribution; naive f32 two-pass summation
+//   src/gpu/sim.rs          RK3 (:135), stage_slots (:1046-1051), record_steps /
+//                           record_nucleation / record_step (:829-916)
+//
+// Templated on the real type: double is the accuracy oracle, float is the
+// reference's own arithmetic.  Compiled with -ffp-contract=off, IEEE '/'
+// and sqrt.  The WGSL `tanh` has implementation-defined accuracy; this file
+// uses expm1-based tanh (Cody-Waite reduction of 2|x| by ln 2, Taylor series
+// of expm1 on the reduced argument), which the tests pin against libm's tanh.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+template <class R> struct P4 { R x, y, z, w; };
+
+template <class R> struct Prm {
+    int nx, ny, nz;
+    R dx, inv_dx, dt, eta, lambda, eps, phi_b, m2, delta, a_rad, wall_width, p_floor, e_floor, z_max, e_ref, t_ref;
+};
+
+// WGSL min/max: min(e1,e2) = e2 if e2 < e1 else e1; max = e2 if e1 < e2 else e1
+template <class R> inline R wmin(R a, R b) { return b < a ? b : a; }
+template <class R> inline R wmax(R a, R b) { return a < b ? b : a; }
+template <class R> inline R wsign(R a) { return a > R(0) ? R(1) : (a < R(0) ? R(-1) : R(0)); }
+template <class R> inline R wdot(const P4<R> &a, const P4<R> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R tiny() { return R(1e-30); }
+template <class R> inline R den_floor() { return R(1e-12); }
+template <> inline float tiny<float>() { return 1e-30f; }
+template <> inline float den_floor<float>() { return 1e-12f; }
+
+// common.wgsl:80-95 (a full modulo; every offset used is within +-2 and the
+// lattice is at least 8 cells along each axis)
+inline int wrap1(int i, int n) {
+    int r = i;
+    if (r < 0) r += n;
+    if (r < 0) r += n;
+    if (r >= n) r -= n;
+    if (r >= n) r -= n;
+    return r;
+}
+template <class R> inline int64_t cell_index(const Prm<R> &P, int x, int y, int z) {
+    return ((int64_t)wrap1(z, P.nz) * P.ny + wrap1(y, P.ny)) * P.nx + wrap1(x, P.nx);
+}
+
+// common.wgsl:114-136
+template <class R> inline R minmod3(R a, R b, R c) {
+    const R sa = wsign(a);
+    if (sa != wsign(b) || sa != wsign(c)) return R(0);
+    return sa * wmin(std::fabs(a), wmin(std::fabs(b), std::fabs(c)));
+}
+template <class R> inline R limit_slope(R dm, R dp) { return minmod3(R(0.5) * (dm + dp), R(2) * dm, R(2) * dp); }
+template <class R> inline P4<R> limit_slope4(const P4<R> &dm, const P4<R> &dp) {
+    return {limit_slope(dm.x, dp.x), limit_slope(dm.y, dp.y), limit_slope(dm.z, dp.z), limit_slope(dm.w, dp.w)};
+}
+
+// potential.wgsl:38-47
+template <class R> inline R potential(const Prm<R> &P, R phi) {
+    const R p2 = phi * phi;
+    return R(0.5) * P.m2 * p2 - P.delta * p2 * phi / R(3) + R(0.25) * P.lambda * p2 * p2;
+}
+template <class R> inline R dpotential(const Prm<R> &P, R phi) {
+    return phi * (P.m2 - P.delta * phi + P.lambda * phi * phi);
+}
+
+// eos.wgsl
+template <class R> inline R eos_sound_speed2(R, R) { return R(1) / R(3); }
+template <class R> inline R eos_enthalpy_from_pressure(R p, R) { return R(4) * p; }
+template <class R> inline R eos_pressure_from_conserved(const Prm<R> &P, R e_lab, R z2, R) {
+    const R disc = wmax(R(4) * e_lab * e_lab - R(3) * z2, R(0));
+    const R p = (e_lab * e_lab - z2) / (e_lab + std::sqrt(disc));
+    return wmax(p, P.p_floor);
+}
+template <class R> inline P4<R> eos_regularise(const Prm<R> &P, const P4<R> &u) {
+    const R e_lab = wmax(u.x, P.e_floor);
+    P4<R> z{u.y, u.z, u.w, R(0)};
+    const R zlen = std::sqrt(wdot(z, z));
+    const R zcap = P.z_max * e_lab;
+    P4<R> zc = z;
+    if (zlen > zcap) {
+        const R f = zcap / wmax(zlen, tiny<R>());
+        zc = {z.x * f, z.y * f, z.z * f, R(0)};
+    }
+    return {e_lab, zc.x, zc.y, zc.z};
+}
+template <class R> inline P4<R> eos_prim_from_cons(const Prm<R> &P, const P4<R> &u_raw, R phi) {
+    const P4<R> u = eos_regularise(P, u_raw);
+    const R e_lab = u.x;
+    const P4<R> z{u.y, u.z, u.w, R(0)};
+    const R p = eos_pressure_from_conserved(P, e_lab, wdot(z, z), phi);
+    const R s = e_lab + p;
+    const R sm = wmax(s, tiny<R>());
+    return {z.x / sm, z.y / sm, z.z / sm, p};
+}
+template <class R> inline P4<R> eos_cons_from_prim(const Prm<R> &P, const P4<R> &q, R phi) {
+    const R v2 = wmin(wdot(q, q), P.z_max * P.z_max);
+    const R w_gam2 = eos_enthalpy_from_pressure(q.w, phi) / (R(1) - v2);
+    return {w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
+}
+template <class R> inline R axis_comp(const P4<R> &v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
+template <class R> inline P4<R> eos_flux(const P4<R> &q, const P4<R> &u, int axis) {
+    const R p = q.w;
+    const R vd = axis_comp(q, axis);
+    P4<R> f{(u.x + p) * vd, u.y * vd, u.z * vd, u.w * vd};
+    if (axis == 0) f.y = f.y + p;
+    if (axis == 1) f.z = f.z + p;
+    if (axis == 2) f.w = f.w + p;
+    return f;
+}
+template <class R> inline void eos_wave_speeds(const Prm<R> &P, const P4<R> &q, R phi, int axis, R &lo, R &hi) {
+    const R v2 = wmin(wdot(q, q), P.z_max * P.z_max);
+    const R vd = axis_comp(q, axis);
+    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs = std::sqrt(cs2);
+    const R denom = R(1) - v2 * cs2;
+    const R rad = wmax((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
+    const R a = vd * (R(1) - cs2) / denom;
+    const R b = cs * std::sqrt(rad) / denom;
+    lo = a - b;
+    hi = a + b;
+}
+
+// step.wgsl:63-90
+template <class R> inline P4<R> sanitise(const Prm<R> &P, const P4<R> &q) {
+    const R p = wmax(q.w, P.p_floor);
+    P4<R> v{q.x, q.y, q.z, R(0)};
+    const R v2 = wdot(v, v);
+    const R vmax2 = P.z_max * P.z_max;
+    if (v2 > vmax2) {
+        const R f = std::sqrt(vmax2 / wmax(v2, tiny<R>()));
+        v = {v.x * f, v.y * f, v.z * f, R(0)};
+    }
+    return {v.x, v.y, v.z, p};
+}
+template <class R>
+inline P4<R> hll_flux(const Prm<R> &P, const P4<R> &q_l, R phi_l, const P4<R> &q_r, R phi_r, int axis) {
+    const P4<R> u_l = eos_cons_from_prim(P, q_l, phi_l);
+    const P4<R> u_r = eos_cons_from_prim(P, q_r, phi_r);
+    const P4<R> f_l = eos_flux(q_l, u_l, axis);
+    const P4<R> f_r = eos_flux(q_r, u_r, axis);
+    R llo, lhi, rlo, rhi;
+    eos_wave_speeds(P, q_l, phi_l, axis, llo, lhi);
+    eos_wave_speeds(P, q_r, phi_r, axis, rlo, rhi);
+    const R s_min = wmin(wmin(llo, rlo), R(0));
+    const R s_max = wmax(wmax(lhi, rhi), R(0));
+    const R den = wmax(s_max - s_min, den_floor<R>());
+    const R smm = s_min * s_max;
+    return {(s_max * f_l.x - s_min * f_r.x + smm * (u_r.x - u_l.x)) / den,
+            (s_max * f_l.y - s_min * f_r.y + smm * (u_r.y - u_l.y)) / den,
+            (s_max * f_l.z - s_min * f_r.z + smm * (u_r.z - u_l.z)) / den,
+            (s_max * f_l.w - s_min * f_r.w + smm * (u_r.w - u_l.w)) / den};
+}
+template <class R> inline P4<R> sub(const P4<R> &a, const P4<R> &b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+template <class R> inline P4<R> addh(const P4<R> &a, const P4<R> &s) {
+    return {a.x + R(0.5) * s.x, a.y + R(0.5) * s.y, a.z + R(0.5) * s.z, a.w + R(0.5) * s.w};
+}
+template <class R> inline P4<R> subh(const P4<R> &a, const P4<R> &s) {
+    return {a.x - R(0.5) * s.x, a.y - R(0.5) * s.y, a.z - R(0.5) * s.z, a.w - R(0.5) * s.w};
+}
+
+// tanh (see header)
+template <class R> struct TK;
+template <> struct TK<float> {
+    static constexpr int terms = 9;
+    static constexpr float hi = 0.693145751953125f, lo = 1.428606765330187045e-06f, inv = 1.44269504088896341f, big = 9.0f;
+};
+template <> struct TK<double> {
+    static constexpr int terms = 16;
+    static constexpr double hi = 6.93147180369123816490e-01, lo = 1.90821492927058770002e-10,
+                            inv = 1.44269504088896338700e+00, big = 19.5;
+};
+template <class R> R o_expm1_pos(R a) {
+    const R kf = std::floor(a * TK<R>::inv + R(0.5));
+    const int k = (int)kf;
+    const R r = (a - kf * TK<R>::hi) - kf * TK<R>::lo;
+    R t = R(1);
+    for (int i = TK<R>::terms; i >= 2; --i) t = R(1) + t * r / R(i);
+    const R em1r = r * t;
+    if (k == 0) return em1r;
+    R f = R(1);
+    for (int i = 0; i < k; ++i) f = f * R(2);
+    return (em1r + R(1)) * f - R(1);
+}
+template <class R> R o_tanh(R x) {
+    const R ax = std::fabs(x);
+    R t;
+    if (ax >= TK<R>::big) t = R(1);
+    else {
+        const R e = o_expm1_pos(R(2) * ax);
+        t = e / (e + R(2));
+    }
+    return x < R(0) ? -t : t;
+}
+
+// nucleate.wgsl:52-57
+template <class R> inline R min_image(R d, R n) {
+    R r = d;
+    if (r > R(0.5) * n) r = r - n;
+    if (r < R(-0.5) * n) r = r + n;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+//  The lattice
+// ---------------------------------------------------------------------------
+struct Event {
+    float time, x, y, z, r0;
+};
+
+template <class R> struct Lattice {
+    Prm<R> P;
+    float dt32;  // the f32 time step of the reference clock
+    int64_t n;
+    std::vector<R> field[3];  // (phi, pi) interleaved
+    std::vector<R> fluid[3];  // (E, Zx, Zy, Zz) interleaved
+    std::vector<R> prim;      // (vx, vy, vz, p)
+    int cur = 0;
+    float time = 0.0f;
+    uint64_t steps = 0;
+    std::vector<Event> schedule;
+    size_t next_event = 0;
+
+    P4<R> fl(int s, int64_t i) const { return {fluid[s][4 * i], fluid[s][4 * i + 1], fluid[s][4 * i + 2], fluid[s][4 * i + 3]}; }
+    P4<R> pr(int64_t i) const { return {prim[4 * i], prim[4 * i + 1], prim[4 * i + 2], prim[4 * i + 3]}; }
+
+    // init.wgsl:18-26 on slot cur
+    void init() {
+        for (int64_t i = 0; i < n; ++i) {
+            field[cur][2 * i] = R(0);
+            field[cur][2 * i + 1] = R(0);
+            fluid[cur][4 * i] = P.e_ref;
+            fluid[cur][4 * i + 1] = R(0);
+            fluid[cur][4 * i + 2] = R(0);
+            fluid[cur][4 * i + 3] = R(0);
+        }
+    }
+
+    // nucleate.wgsl:59-85 on slot cur
+    void nucleate(const std::vector<Event> &batch) {
+        const R nx = R(P.nx), ny = R(P.ny), nz = R(P.nz);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const int x = (int)(i % P.nx), y = (int)((i / P.nx) % P.ny), z = (int)(i / ((int64_t)P.nx * P.ny));
+            const R px = R(x), py = R(y), pz = R(z);
+            R phi = field[cur][2 * i];
+            for (const Event &b : batch) {
+                const P4<R> d{min_image(px - R(b.x), nx), min_image(py - R(b.y), ny), min_image(pz - R(b.z), nz), R(0)};
+                const R r = std::sqrt(wdot(d, d)) * P.dx;
+                const R profile = R(0.5) * P.phi_b * (R(1) - o_tanh((r - R(b.r0) * P.dx) / P.wall_width));
+                phi = wmax(phi, profile);
+            }
+            field[cur][2 * i] = phi;
+        }
+    }
+
+    // primitives.wgsl:16-23
+    void primitives(int in) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const P4<R> q = eos_prim_from_cons(P, fl(in, i), field[in][2 * i]);
+            prim[4 * i] = q.x;
+            prim[4 * i + 1] = q.y;
+            prim[4 * i + 2] = q.z;
+            prim[4 * i + 3] = q.w;
+        }
+    }
+
+    // step.wgsl:92-174
+    void step_pass(int sn, int in, int out, R a0, R a1, R dt_stage) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i0 = 0; i0 < n; ++i0) {
+            const int cx = (int)(i0 % P.nx), cy = (int)((i0 / P.nx) % P.ny), cz = (int)(i0 / ((int64_t)P.nx * P.ny));
+            const R phi0 = field[in][2 * i0], pi0 = field[in][2 * i0 + 1];
+            const P4<R> q0 = sanitise(P, pr(i0));
+            R lap_sum = R(0);
+            R grad[3] = {R(0), R(0), R(0)};
+            P4<R> flux_div{R(0), R(0), R(0), R(0)};
+            for (int axis = 0; axis < 3; ++axis) {
+                const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+                const int64_t im1 = cell_index(P, cx - ex, cy - ey, cz - ez);
+                const int64_t ip1 = cell_index(P, cx + ex, cy + ey, cz + ez);
+                const int64_t im2 = cell_index(P, cx - 2 * ex, cy - 2 * ey, cz - 2 * ez);
+                const int64_t ip2 = cell_index(P, cx + 2 * ex, cy + 2 * ey, cz + 2 * ez);
+                const R phi_m = field[in][2 * im1];
+                const R phi_p = field[in][2 * ip1];
+                lap_sum = lap_sum + (phi_m + phi_p - R(2) * phi0);
+                grad[axis] = R(0.5) * (phi_p - phi_m) * P.inv_dx;
+                const P4<R> qm2 = sanitise(P, pr(im2));
+                const P4<R> qm1 = sanitise(P, pr(im1));
+                const P4<R> qp1 = sanitise(P, pr(ip1));
+                const P4<R> qp2 = sanitise(P, pr(ip2));
+                const P4<R> slope_m1 = limit_slope4(sub(qm1, qm2), sub(q0, qm1));
+                const P4<R> slope_0 = limit_slope4(sub(q0, qm1), sub(qp1, q0));
+                const P4<R> slope_p1 = limit_slope4(sub(qp1, q0), sub(qp2, qp1));
+                const P4<R> f_minus =
+                    hll_flux(P, sanitise(P, addh(qm1, slope_m1)), phi_m, sanitise(P, subh(q0, slope_0)), phi0, axis);
+                const P4<R> f_plus =
+                    hll_flux(P, sanitise(P, addh(q0, slope_0)), phi0, sanitise(P, subh(qp1, slope_p1)), phi_p, axis);
+                flux_div = {flux_div.x + (f_plus.x - f_minus.x) * P.inv_dx, flux_div.y + (f_plus.y - f_minus.y) * P.inv_dx,
+                            flux_div.z + (f_plus.z - f_minus.z) * P.inv_dx, flux_div.w + (f_plus.w - f_minus.w) * P.inv_dx};
+            }
+            const R laplacian = lap_sum * P.inv_dx * P.inv_dx;
+            const P4<R> gv{grad[0], grad[1], grad[2], R(0)};
+            const R gam = R(1) / std::sqrt(wmax(R(1) - wdot(q0, q0), den_floor<R>()));
+            const R dphi_proper = gam * (pi0 + wdot(q0, gv));
+            const R drag = P.eta * dphi_proper;
+            const R d_phi = pi0;
+            const R d_pi = laplacian - dpotential(P, phi0) - drag;
+            P4<R> d_fluid{-flux_div.x, -flux_div.y, -flux_div.z, -flux_div.w};
+            d_fluid.x = d_fluid.x + drag * pi0;
+            d_fluid.y = d_fluid.y - drag * grad[0];
+            d_fluid.z = d_fluid.z - drag * grad[1];
+            d_fluid.w = d_fluid.w - drag * grad[2];
+            const R fu_phi = phi0 + dt_stage * d_phi;
+            const R fu_pi = pi0 + dt_stage * d_pi;
+            const P4<R> uin = fl(in, i0);
+            const P4<R> fu{uin.x + dt_stage * d_fluid.x, uin.y + dt_stage * d_fluid.y, uin.z + dt_stage * d_fluid.z,
+                           uin.w + dt_stage * d_fluid.w};
+            const R fn_phi = field[sn][2 * i0], fn_pi = field[sn][2 * i0 + 1];
+            const P4<R> un = fl(sn, i0);
+            field[out][2 * i0] = a0 * fn_phi + a1 * fu_phi;
+            field[out][2 * i0 + 1] = a0 * fn_pi + a1 * fu_pi;
+            fluid[out][4 * i0] = a0 * un.x + a1 * fu.x;
+            fluid[out][4 * i0 + 1] = a0 * un.y + a1 * fu.y;
+            fluid[out][4 * i0 + 2] = a0 * un.z + a1 * fu.z;
+            fluid[out][4 * i0 + 3] = a0 * un.w + a1 * fu.w;
+        }
+    }
+
+    // record_step (sim.rs:881-916) with RK3 (sim.rs:135) and stage_slots
+    // (sim.rs:1046-1051)
+    void record_step() {
+        const float rk32[3][2] = {{0.0f, 1.0f}, {0.75f, 0.25f}, {1.0f / 3.0f, 2.0f / 3.0f}};
+        const int a = cur, b = (cur + 1) % 3, c = (cur + 2) % 3;
+        const int slots[3][2] = {{a, b}, {b, c}, {c, b}};
+        for (int s = 0; s < 3; ++s) {
+            const int in = slots[s][0], out = slots[s][1];
+            primitives(in);
+            step_pass(cur, in, out, R(rk32[s][0]), R(rk32[s][1]), P.dt);
+        }
+        cur = (cur + 1) % 3;
+        time += dt32;
+        steps += 1;
+    }
+
+    // record_nucleation (sim.rs:843-879)
+    bool record_nucleation() {
+        std::vector<Event> batch;
+        while (next_event < schedule.size() && schedule[next_event].time <= time && batch.size() < 1024) {
+            batch.push_back(schedule[next_event]);
+            ++next_event;
+        }
+        if (batch.empty()) return false;
+        nucleate(batch);
+        return true;
+    }
+};
+
+// reduce.wgsl:56-79 for cell i of slot s (the primitive cache holds slot s)
+template <class R> inline void cell_contribution(const Lattice<R> &L, int s, int64_t i, R out[4], R &vabs) {
+    const Prm<R> &P = L.P;
+    const int x = (int)(i % P.nx), y = (int)((i / P.nx) % P.ny), z = (int)(i / ((int64_t)P.nx * P.ny));
+    const R phi = L.field[s][2 * i], pi = L.field[s][2 * i + 1];
+    R grad2 = R(0);
+    for (int axis = 0; axis < 3; ++axis) {
+        const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+        const R d = R(0.5) * (L.field[s][2 * cell_index(P, x + ex, y + ey, z + ez)] -
+                              L.field[s][2 * cell_index(P, x - ex, y - ey, z - ez)]) * P.inv_dx;
+        grad2 = grad2 + d * d;
+    }
+    const R e_lab = wmax(L.fluid[s][4 * i], P.e_floor);
+    const P4<R> q = L.pr(i);
+    const R v2 = wdot(q, q);
+    out[0] = R(0.5) * pi * pi + R(0.5) * grad2 + potential(P, phi) + e_lab;
+    out[1] = phi > R(0.5) * P.phi_b ? R(1) : R(0);
+    out[2] = (e_lab + q.w) * v2;
+    out[3] = e_lab;
+    vabs = std::sqrt(v2);
+}
+
+template <class R> Prm<R> make_prm(const float *p32) {
+    // p32 holds the 96-byte SimParams block as 24 words (sim.rs:66-98)
+    uint32_t u[4];
+    std::memcpy(u, p32, 16);
+    Prm<R> P;
+    P.nx = (int)u[0]; P.ny = (int)u[1]; P.nz = (int)u[2];
+    P.dx = p32[4]; P.inv_dx = p32[5]; P.dt = p32[6]; P.eta = p32[7];
+    P.lambda = p32[8]; P.eps = p32[9]; P.phi_b = p32[10]; P.m2 = p32[11];
+    P.delta = p32[12]; P.a_rad = p32[13]; P.wall_width = p32[14]; P.p_floor = p32[15];
+    P.e_floor = p32[16]; P.z_max = p32[17]; P.e_ref = p32[18]; P.t_ref = p32[19];
+    return P;
+}
+
+struct Handle {
+    int precision;  // 0 f32, 1 f64
+    Lattice<float> *lf = nullptr;
+    Lattice<double> *ld = nullptr;
+};
+
+template <class R> Lattice<R> *make_lattice(const float *params, const float *events, int nev) {
+    Lattice<R> *L = new Lattice<R>();
+    L->P = make_prm<R>(params);
+    L->dt32 = params[6];
+    L->n = (int64_t)L->P.nx * L->P.ny * L->P.nz;
+    for (int s = 0; s < 3; ++s) {
+        L->field[s].assign(2 * L->n, R(0));
+        L->fluid[s].assign(4 * L->n, R(0));
+    }
+    L->prim.assign(4 * L->n, R(0));
+    for (int k = 0; k < nev; ++k) {
+        const float *e = events + 5 * k;
+        L->schedule.push_back(Event{e[0], e[1], e[2], e[3], e[4]});
+    }
+    L->init();
+    return L;
+}
+
+template <class R> void diag_exact(Lattice<R> &L, double *out) {
+    // per-cell contributions in R; sums in long double
+    L.primitives(L.cur);
+    long double acc[4] = {0, 0, 0, 0};
+    double vmax = 0.0;
+    for (int64_t i = 0; i < L.n; ++i) {
+        R c[4], va;
+        cell_contribution(L, L.cur, i, c, va);
+        for (int k = 0; k < 4; ++k) acc[k] += (long double)c[k];
+        if ((double)va > vmax) vmax = (double)va;
+    }
+    for (int k = 0; k < 4; ++k) out[k] = (double)acc[k];
+    out[4] = vmax;
+}
+
+// reduce.wgsl:81-140 with its exact f32 summation order: 1024 groups x 256
+// threads grid-striding, shared-memory halving tree, one-group second pass
+void diag_naive_f32(Lattice<float> &L, float *out4, uint32_t *vmax_bits) {
+    L.primitives(L.cur);
+    const uint32_t NP = 1024, WG = 256, stride = NP * WG;
+    std::vector<float> partials(4 * NP);
+    uint32_t maxbits = 0;
+    std::vector<float> sh(4 * WG), shm(WG);
+    for (uint32_t wid = 0; wid < NP; ++wid) {
+        for (uint32_t lid = 0; lid < WG; ++lid) {
+            float acc[4] = {0, 0, 0, 0};
+            float vmax = 0.0f;
+            for (uint64_t i = (uint64_t)wid * WG + lid; i < (uint64_t)L.n; i += stride) {
+                float c[4], va;
+                cell_contribution(L, L.cur, (int64_t)i, c, va);
+                for (int k = 0; k < 4; ++k) acc[k] = acc[k] + c[k];
+                vmax = wmax(vmax, va);
+            }
+            for (int k = 0; k < 4; ++k) sh[4 * lid + k] = acc[k];
+            shm[lid] = vmax;
+        }
+        for (uint32_t s = WG / 2; s > 0; s >>= 1)
+            for (uint32_t lid = 0; lid < s; ++lid) {
+                for (int k = 0; k < 4; ++k) sh[4 * lid + k] = sh[4 * lid + k] + sh[4 * (lid + s) + k];
+                shm[lid] = wmax(shm[lid], shm[lid + s]);
+            }
+        for (int k = 0; k < 4; ++k) partials[4 * wid + k] = sh[k];
+        uint32_t b;
+        std::memcpy(&b, &shm[0], 4);
+        if (b > maxbits) maxbits = b;
+    }
+    for (uint32_t lid = 0; lid < WG; ++lid) {
+        float acc[4] = {0, 0, 0, 0};
+        for (uint32_t i = lid; i < NP; i += WG)
+            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + partials[4 * i + k];
+        for (int k = 0; k < 4; ++k) sh[4 * lid + k] = acc[k];
+    }
+    for (uint32_t s = WG / 2; s > 0; s >>= 1)
+        for (uint32_t lid = 0; lid < s; ++lid)
+            for (int k = 0; k < 4; ++k) sh[4 * lid + k] = sh[4 * lid + k] + sh[4 * (lid + s) + k];
+    for (int k = 0; k < 4; ++k) out4[k] = sh[k];
+    *vmax_bits = maxbits;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+//  ctypes surface
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// params: 24 words of the SimParams block; events: 5 floats each
+// (time, x, y, z, r0), already sorted by time.
+void *oracle_create(int precision, const float *params, const float *events, int nev) {
+    Handle *h = new Handle();
+    h->precision = precision;
+    if (precision == 1) h->ld = make_lattice<double>(params, events, nev);
+    else h->lf = make_lattice<float>(params, events, nev);
+    return h;
+}
+
+void oracle_destroy(void *hp) {
+    Handle *h = (Handle *)hp;
+    delete h->lf;
+    delete h->ld;
+    delete h;
+}
+
+// headless::advance's stepping (headless.rs:126-131): n x (nucleate due
+// events, one RK3 step).  Returns the number of steps that stamped bubbles.
+int oracle_advance(void *hp, long long nsteps) {
+    Handle *h = (Handle *)hp;
+    int stamped = 0;
+    for (long long k = 0; k < nsteps; ++k) {
+        if (h->ld) {
+            stamped += h->ld->record_nucleation();
+            h->ld->record_step();
+        } else {
+            stamped += h->lf->record_nucleation();
+            h->lf->record_step();
+        }
+    }
+    return stamped;
+}
+
+// one record_steps(n) (sim.rs:829-841): nucleate once, then n steps
+int oracle_record_steps(void *hp, long long nsteps) {
+    Handle *h = (Handle *)hp;
+    int stamped = h->ld ? h->ld->record_nucleation() : h->lf->record_nucleation();
+    for (long long k = 0; k < nsteps; ++k) {
+        if (h->ld) h->ld->record_step();
+        else h->lf->record_step();
+    }
+    return stamped;
+}
+
+int oracle_remaining(void *hp) {
+    Handle *h = (Handle *)hp;
+    return h->ld ? (int)(h->ld->schedule.size() - h->ld->next_event) : (int)(h->lf->schedule.size() - h->lf->next_event);
+}
+
+float oracle_time(void *hp) {
+    Handle *h = (Handle *)hp;
+    return h->ld ? h->ld->time : h->lf->time;
+}
+
+// state of slot cur; field/fluid buffers in the lattice precision
+void oracle_get_state(void *hp, void *field, void *fluid) {
+    Handle *h = (Handle *)hp;
+    if (h->ld) {
+        std::memcpy(field, h->ld->field[h->ld->cur].data(), h->ld->field[0].size() * 8);
+        std::memcpy(fluid, h->ld->fluid[h->ld->cur].data(), h->ld->fluid[0].size() * 8);
+    } else {
+        std::memcpy(field, h->lf->field[h->lf->cur].data(), h->lf->field[0].size() * 4);
+        std::memcpy(fluid, h->lf->fluid[h->lf->cur].data(), h->lf->fluid[0].size() * 4);
+    }
+}
+
+void oracle_set_state(void *hp, const void *field, const void *fluid) {
+    Handle *h = (Handle *)hp;
+    if (h->ld) {
+        std::memcpy(h->ld->field[h->ld->cur].data(), field, h->ld->field[0].size() * 8);
+        std::memcpy(h->ld->fluid[h->ld->cur].data(), fluid, h->ld->fluid[0].size() * 8);
+    } else {
+        std::memcpy(h->lf->field[h->lf->cur].data(), field, h->lf->field[0].size() * 4);
+        std::memcpy(h->lf->fluid[h->lf->cur].data(), fluid, h->lf->fluid[0].size() * 4);
+    }
+}
+
+// sums of (e_total, broken, kinetic, E) and max|v| -> out[5]; contributions
+// in the lattice precision, summed in long double
+void oracle_diagnostics(void *hp, double *out) {
+    Handle *h = (Handle *)hp;
+    if (h->ld) diag_exact(*h->ld, out);
+    else diag_exact(*h->lf, out);
+}
+
+// reduce.wgsl's own f32 result (f32 lattices only)
+void oracle_diagnostics_naive_f32(void *hp, float *sums, uint32_t *vmax_bits) {
+    Handle *h = (Handle *)hp;
+    if (h->lf) diag_naive_f32(*h->lf, sums, vmax_bits);
+}
+
+// --- elementary functions, f64 and f32, for unit tests -------------------
+// params: the 24-word SimParams block
+void oracle_prim_from_cons(int precision, const float *params, const double *u, double phi, double *q) {
+    if (precision == 1) {
+        const Prm<double> P = make_prm<double>(params);
+        const P4<double> r = eos_prim_from_cons(P, P4<double>{u[0], u[1], u[2], u[3]}, phi);
+        q[0] = r.x; q[1] = r.y; q[2] = r.z; q[3] = r.w;
+    } else {
+        const Prm<float> P = make_prm<float>(params);
+        const P4<float> r = eos_prim_from_cons(P, P4<float>{(float)u[0], (float)u[1], (float)u[2], (float)u[3]}, (float)phi);
+        q[0] = r.x; q[1] = r.y; q[2] = r.z; q[3] = r.w;
+    }
+}
+
+void oracle_cons_from_prim(const float *params, const double *q, double *u) {
+    const Prm<double> P = make_prm<double>(params);
+    const P4<double> r = eos_cons_from_prim(P, P4<double>{q[0], q[1], q[2], q[3]}, 0.0);
+    u[0] = r.x; u[1] = r.y; u[2] = r.z; u[3] = r.w;
+}
+
+void oracle_wave_speeds(const float *params, const double *q, int axis, double *lohi) {
+    const Prm<double> P = make_prm<double>(params);
+    eos_wave_speeds(P, P4<double>{q[0], q[1], q[2], q[3]}, 0.0, axis, lohi[0], lohi[1]);
+}
+
+void oracle_hll_flux(const float *params, const double *ql, const double *qr, int axis, double *f) {
+    const Prm<double> P = make_prm<double>(params);
+    const P4<double> r = hll_flux(P, P4<double>{ql[0], ql[1], ql[2], ql[3]}, 0.0, P4<double>{qr[0], qr[1], qr[2], qr[3]}, 0.0, axis);
+    f[0] = r.x; f[1] = r.y; f[2] = r.z; f[3] = r.w;
+}
+
+void oracle_eos_flux(const double *q, const double *u, int axis, double *f) {
+    const P4<double> r = eos_flux(P4<double>{q[0], q[1], q[2], q[3]}, P4<double>{u[0], u[1], u[2], u[3]}, axis);
+    f[0] = r.x; f[1] = r.y; f[2] = r.z; f[3] = r.w;
+}
+
+double oracle_limit_slope(double dm, double dp) { return limit_slope(dm, dp); }
+float oracle_limit_slope_f32(float dm, float dp) { return limit_slope(dm, dp); }
+
+double oracle_tanh_f64(double x) { return o_tanh(x); }
+float oracle_tanh_f32(float x) { return o_tanh(x); }
+
+double oracle_potential(const float *params, double phi) { return potential(make_prm<double>(params), phi); }
+double oracle_dpotential(const float *params, double phi) { return dpotential(make_prm<double>(params), phi); }
+
+}  // extern "C"
