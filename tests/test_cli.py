@@ -1,0 +1,84 @@
+"""The bubble-sim front end: the reference's flags (config.rs:13-115), its
+validation messages (config.rs:151-178, physics.rs:191-217) and the headless
+report (headless.rs:23-116)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+
+def cli(bs, *args, timeout=300):
+    return subprocess.run([bs.CLI_PATH, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_lists_reference_flags(bs):
+    r = cli(bs, "--help")
+    assert r.returncode == 0
+    for flag in ("--grid", "--lambda", "--eps-ratio", "--alpha", "--eta", "--cfl", "--seed-factor", "--seed-radius",
+                 "--bubbles", "--nucleation", "--nucleation-duration", "--seed", "--steps-per-frame", "--field",
+                 "--no-vsync", "--headless", "--report-every", "--screenshot", "--screenshot-size"):
+        assert flag in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--lambda", "50", "--headless", "1"], "cells thick, which the lattice cannot resolve"),
+    (["--cfl", "0.6", "--headless", "1"], "cfl must lie in (0, 0.5]"),
+    (["--eps-ratio", "1.0", "--headless", "1"], "eps_ratio must lie in (0, 1)"),
+    (["--bubbles", "0", "--headless", "1"], "--bubbles must be at least 1"),
+    (["--grid", "64,7,64", "--headless", "1"], "grid axis 1 is 7"),
+    (["--nucleation-duration", "0", "--headless", "1"], "--nucleation-duration must be positive"),
+    (["--steps-per-frame", "0", "--headless", "1"], "--steps-per-frame must be at least 1"),
+])
+def test_validation_errors(bs, args, msg):
+    r = cli(bs, *args)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+@pytest.mark.parametrize("args", [["--grid", "1,2"], ["--grid", "abc"], ["--nucleation", "often"], ["--bogus"],
+                                  ["--lambda"]])
+def test_usage_errors(bs, args):
+    r = cli(bs, *args)
+    assert r.returncode == 2
+    assert "error:" in r.stderr
+
+
+def test_interactive_and_screenshot_are_out_of_scope(bs):
+    r = cli(bs)
+    assert r.returncode == 1 and "--headless" in r.stderr
+    r = cli(bs, "--headless", "10", "--screenshot", "x.png")
+    assert r.returncode == 1 and "renderer" in r.stderr
+
+
+ROW = re.compile(r"^ {0,9}\d+ {1,10}\d+\.\d {1,9}\d+\.\d\d% {1,10}\d+\.\d {1,11}\d\.\d{4} {1,11}-?\d\.\d{3}e-?\d+ +"
+                 r"(-|\d\.\d{6})$")
+
+
+@pytest.mark.gpu
+def test_headless_report_format(bs, gpu_available, tmp_path):
+    dump = tmp_path / "state.bin"
+    r = cli(bs, "--grid", "48", "--bubbles", "1", "--nucleation", "simultaneous", "--headless", "40",
+            "--report-every", "10", "--dump", str(dump))
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("adapter: ")
+    assert re.match(r"^lattice 48x48x48 = 0\.11 M cells \| \d+\.\d\d GB \| dt = 0\.200$", lines[1])
+    assert lines[2] == "wall 4.00 cells | R_c 13.3 cells | sigma 0.0833 | eps 0.0125 | e_ref 0.1250"
+    assert lines[4] == "      step       time     broken       R_eq      max|v|           K       E/E0"
+    rows = lines[5:9]
+    for row in rows:
+        assert ROW.match(row), row
+    assert rows[0].split()[0] == "10" and rows[-1].split()[0] == "40"
+    assert rows[0].split()[-1] == "1.000000"  # the only bubble is in at step 0
+    assert re.match(r"^40 steps in \d+\.\d\d s  \(\d+\.\d steps/s, \d+\.\d\d G cell-updates/s\)$", lines[-1])
+    assert os.path.getsize(dump) == 48 ** 3 * 6 * 4
+
+
+@pytest.mark.gpu
+def test_default_report_every(bs, gpu_available):
+    r = cli(bs, "--grid", "32", "--headless", "40")
+    assert r.returncode == 0
+    rows = [l for l in r.stdout.splitlines() if ROW.match(l)]
+    assert len(rows) == 20  # max(steps / 20, 1) = 2 -> 20 reports (main.rs:32)

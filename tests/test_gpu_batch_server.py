@@ -1,0 +1,137 @@
+"""Batched sweep and served jobs: every box gives the bits it gives alone.
+
+* Batch (one fused launch per RK stage across uneven boxes, device-driven
+  nucleation, hipGraph replay) against one Simulation per box: bitwise under
+  ExactMath (the per-box arithmetic is the same sequence), and graph replay
+  bitwise equal to eager stepping under FastMath.
+* Server (continuous batching under an HBM budget): each job's final state
+  checksum is identical when it runs alone and inside a mix of concurrently
+  admitted jobs; ExactMath jobs equal a plain Simulation bitwise.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs(bs, math, n=6, seed=3):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n):
+        edge = int(rng.choice([16, 24, 32, 40]))
+        grid = (edge, int(rng.choice([16, 24, 32])), int(rng.choice([16, 20, 32])))
+        eta = float(np.exp(np.log(0.02) + rng.uniform() * (0 - np.log(0.02))))
+        model = bs.Model(eta=eta, seed_factor=float(rng.uniform(0.7, 1.5)), eps_ratio=float(rng.uniform(0.6, 0.99)))
+        specs.append(bs.make_spec(grid=grid, model=model, nucleation=bs.NUC_UNIFORM_STEPS,
+                                  bubbles=int(rng.integers(1, 5)), seed=k + 1, uniform_max_step=8, time_seed=100 + k,
+                                  math=math))
+    return specs
+
+
+def test_batch_equals_single_boxes_exact(bs, gpu_available):
+    specs = _specs(bs, bs.MATH_EXACT)
+    b = bs.Batch(specs)
+    b.advance(6, use_graph=False)
+    b.advance(9, use_graph=True)
+    for i, s in enumerate(specs):
+        print(f"box {i} grid {tuple(s.grid)}", flush=True)
+        sim = bs.Simulation(s)
+        sim.advance(15)
+        f, u = sim.download()
+        bf, bu = b.download(i)
+        assert b.steps(i) == 15
+        assert np.array_equal(f, bf) and np.array_equal(u, bu)
+        d1, d2 = sim.diagnostics(), b.diagnostics(i)
+        assert d1.mean_energy == d2.mean_energy and d1.max_velocity == d2.max_velocity
+        sim.close()
+    b.close()
+
+
+def test_batch_graph_equals_eager_fast(bs, gpu_available):
+    specs = _specs(bs, bs.MATH_FAST, n=5, seed=9)
+    a = bs.Batch(specs)
+    a.advance(60, use_graph=True)
+    c = bs.Batch(specs)
+    c.advance(60, use_graph=False)
+    for i in range(len(specs)):
+        fa, ua = a.download(i)
+        fc, uc = c.download(i)
+        assert np.array_equal(fa, fc) and np.array_equal(ua, uc)
+        assert np.isfinite(ua).all()
+    a.close()
+    c.close()
+
+
+def _wait_all(srv, ids, timeout=300):
+    return {j: srv.wait(j, timeout) for j in ids}
+
+
+def test_server_alone_equals_mixed(bs, gpu_available):
+    specs = _specs(bs, bs.MATH_FAST, n=6, seed=5)
+    steps = [30, 45, 20, 60, 35, 50]
+    srv = bs.Server(budget_bytes=2 << 30, chunk_steps=5)
+    alone = {}
+    for k, (s, n) in enumerate(zip(specs, steps)):
+        r = srv.wait(srv.submit(s, n), 300)
+        assert r.status == 0 and r.steps == n
+        alone[k] = r.state_checksum
+    ids = [srv.submit(s, n) for s, n in zip(specs, steps)]
+    res = _wait_all(srv, ids)
+    for k, j in enumerate(ids):
+        assert res[j].status == 0 and res[j].steps == steps[k]
+        assert res[j].state_checksum == alone[k], f"job {k} differs in the mix"
+    st = srv.stats()
+    assert st.completed == 12 and st.rejected == 0
+    srv.close()
+
+
+def test_server_exact_equals_simulation(bs, gpu_available):
+    specs = _specs(bs, bs.MATH_EXACT, n=3, seed=11)
+    srv = bs.Server(budget_bytes=1 << 30, chunk_steps=4, math=bs.MATH_EXACT)
+    ids = [srv.submit(s, 13, keep_state=True) for s in specs]
+    res = _wait_all(srv, ids)
+    for s, j in zip(specs, ids):
+        assert res[j].status == 0
+        f, u = srv.fetch_state(j, s)
+        sim = bs.Simulation(s)
+        sim.advance(13)
+        sf, su = sim.download()
+        assert np.array_equal(f, sf) and np.array_equal(u, su)
+        sim.close()
+    srv.close()
+
+
+def test_server_admission_and_rejection(bs, gpu_available):
+    s32 = bs.make_spec(grid=32, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1)
+    one = bs.lattice_bytes((32, 32, 32))
+    # room for exactly one 32^3 job at a time: jobs run one after another
+    srv = bs.Server(budget_bytes=one + one // 2, chunk_steps=5)
+    ids = [srv.submit(s32, 20) for _ in range(3)]
+    res = _wait_all(srv, ids)
+    fin = sorted((res[j].admit_s, res[j].finish_s) for j in ids)
+    for (a0, f0), (a1, f1) in zip(fin, fin[1:]):
+        assert a1 >= f0 - 1e-3  # admitted only after the previous one freed its memory
+    with pytest.raises(bs.BubbleSimError) as e:
+        srv.submit(bs.make_spec(grid=64), 10)
+    assert e.value.code == bs.BS_ERR_OOM
+    with pytest.raises(bs.BubbleSimError) as e:
+        srv.submit(bs.make_spec(grid=32, model=bs.Model(cfl=0.9)), 10)
+    assert e.value.code == bs.BS_ERR_INVALID
+    assert srv.stats().rejected == 2
+    srv.close()
+
+
+def test_server_overlaps_jobs(bs, gpu_available):
+    # continuous batching: a job submitted while another runs is admitted and
+    # advanced in the same launches, not queued behind it
+    srv = bs.Server(budget_bytes=4 << 30, chunk_steps=5)
+    big = srv.submit(bs.make_spec(grid=96, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1), 400)
+    time.sleep(0.05)
+    small = srv.submit(bs.make_spec(grid=32, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1), 40)
+    rs = srv.wait(small, 300)
+    rb = srv.wait(big, 300)
+    assert rs.finish_s < rb.finish_s
+    assert rs.admit_s < rb.finish_s
+    srv.close()
