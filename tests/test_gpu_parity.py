@@ -94,6 +94,11 @@ def _rel_l2(a, b):
 
 @pytest.mark.parametrize("kernel", [0, 1])
 def test_fast_close_to_exact_and_f64(bs, gpu_available, kernel):
+    # Error norm: relative L2 over the box of each state block -- (phi, pi)
+    # and (E, Zx, Zy, Zz) -- against the f64 solver, bar 1e-5.  Per component,
+    # the fast path is no worse than 1.5x the reference's own f32 arithmetic
+    # (the exact f32 path): a nearly-zero component such as Zx early on is
+    # dominated by f32 roundoff relative to its own small norm.
     grid = (48, 48, 48)
     sched = _sched(bs, grid, [(0.0, 24.0, 24.0, 24.0), (1.0, 5.0, 9.0, 30.0)])
     out = {}
@@ -104,15 +109,19 @@ def test_fast_close_to_exact_and_f64(bs, gpu_available, kernel):
         sim.advance(20)
         out[name] = sim.download()
     for k in range(2):
-        for c in range(out["f64"][k].shape[-1]):
-            ref = out["f64"][k][..., c]
-            if np.sum(ref.astype(np.float64) ** 2) == 0:
+        ref = out["f64"][k]
+        e_fast = _rel_l2(out["fast"][k], ref)
+        e_exact = _rel_l2(out["exact"][k], ref)
+        _log(f"block {k}: fast-vs-f64 {e_fast:.3e}  exact-f32-vs-f64 {e_exact:.3e}")
+        assert e_fast < 1e-5 and e_exact < 1e-5
+        for c in range(ref.shape[-1]):
+            if np.sum(ref[..., c].astype(np.float64) ** 2) == 0:
+                assert np.all(out["fast"][k][..., c] == 0)
                 continue
-            e_fast = _rel_l2(out["fast"][k][..., c], out["exact"][k][..., c])
-            e_f64 = _rel_l2(out["exact"][k][..., c], ref)
-            _log(f"array {k} comp {c}: fast-vs-exact {e_fast:.3e}  f32-vs-f64 {e_f64:.3e}")
-            assert e_fast < 1e-5
-            assert e_f64 < 1e-5
+            ef = _rel_l2(out["fast"][k][..., c], ref[..., c])
+            ee = _rel_l2(out["exact"][k][..., c], ref[..., c])
+            _log(f"  comp {c}: fast-vs-f64 {ef:.3e}  exact-f32-vs-f64 {ee:.3e}")
+            assert ef <= 1.5 * ee + 1e-7
 
 
 def test_fused_equals_two_pass_fast_small_tiles(bs, gpu_available):
