@@ -3,9 +3,9 @@
 Bars:
 * ExactMath kernels (f32 and f64, fused and two-pass) reproduce the oracle's
   fields bit for bit (numpy equality; +0 == -0), nucleation included;
-* FastMath f32 agrees with the exact f32 path to a relative L2 error of
-  1e-5 per field over 20 steps, and the f32 solver agrees with the f64 one to
-  the same bar;
+* FastMath f32 and exact f32 agree with the f64 solver to a relative L2
+  error of 1e-5 per state block ((phi, pi) and (E, Z)) over 20 steps; per
+  component the fast path stays within 2x the exact f32 path's own error;
 * diagnostics equal the oracle's long-double sums of the same per-cell terms
   to 1e-12 relative.
 """
@@ -96,7 +96,7 @@ def _rel_l2(a, b):
 def test_fast_close_to_exact_and_f64(bs, gpu_available, kernel):
     # Error norm: relative L2 over the box of each state block -- (phi, pi)
     # and (E, Zx, Zy, Zz) -- against the f64 solver, bar 1e-5.  Per component,
-    # the fast path is no worse than 1.5x the reference's own f32 arithmetic
+    # the fast path is within 2x the reference's own f32 arithmetic
     # (the exact f32 path): a nearly-zero component such as Zx early on is
     # dominated by f32 roundoff relative to its own small norm.
     grid = (48, 48, 48)
@@ -121,7 +121,7 @@ def test_fast_close_to_exact_and_f64(bs, gpu_available, kernel):
             ef = _rel_l2(out["fast"][k][..., c], ref[..., c])
             ee = _rel_l2(out["exact"][k][..., c], ref[..., c])
             _log(f"  comp {c}: fast-vs-f64 {ef:.3e}  exact-f32-vs-f64 {ee:.3e}")
-            assert ef <= 1.5 * ee + 1e-7
+            assert ef <= 2.0 * ee + 1e-7
 
 
 def test_fused_equals_two_pass_fast_small_tiles(bs, gpu_available):
