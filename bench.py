@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "bubble-simulation_amd"), ROOT]
 
@@ -67,6 +69,113 @@ def cpu_baseline(bs, budget_s=12.0):
             "sample": f"{n}^3 box of the same physics, {steps} RK3 steps in {dt:.1f} s, OpenMP {threads} threads"}
 
 
+def sweep_specs(bs, n_boxes=64, master_seed=2024):
+    """Batched-sweep boxes: edge U{32,48,64,96,128}, eta log-U[0.02,1],
+    seed_factor U[0.7,1.5], eps_ratio U[0.6,1.0), 1-4 bubbles at seeded
+    positions (seed k+1), nucleation steps U[0,200]."""
+    u = bs.pcg64mcg_f32(master_seed, 6 * n_boxes)
+    specs = []
+    for k in range(n_boxes):
+        a, b, c, d, e, f = (float(v) for v in u[6 * k:6 * k + 6])
+        edge = [32, 48, 64, 96, 128][min(int(a * 5), 4)]
+        eta = math.exp(math.log(0.02) + b * (0.0 - math.log(0.02)))
+        model = bs.Model(eta=eta, seed_factor=0.7 + 0.8 * c, eps_ratio=min(0.6 + 0.4 * d, 0.999))
+        specs.append(bs.make_spec(grid=edge, model=model, nucleation=bs.NUC_UNIFORM_STEPS, bubbles=1 + min(int(e * 4), 3),
+                                  seed=k + 1, uniform_max_step=200, time_seed=1000 + k))
+    return specs
+
+
+def batched_sweep(bs, steps=200, seq_steps=40):
+    """64 uneven boxes in one launch per RK stage (hipGraph replay) against
+    the same boxes solved one after another on the plain path."""
+    specs = sweep_specs(bs)
+    b = bs.Batch(specs)
+    cells = b.total_cells()
+    b.advance(25, use_graph=True)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.advance(steps, use_graph=True)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    tile = b.tile()
+    b.close()
+    batched = cells * steps / dt
+    log(f"bench: batched {len(specs)} boxes, {cells / 1e6:.1f} M cells, {steps} steps in {dt:.2f} s "
+        f"= {batched / 1e9:.2f} G/s (tile {tile})")
+    seq_t, seq_work = 0.0, 0
+    for s in specs:
+        sim = bs.Simulation(s)
+        sim.advance(5)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance(seq_steps)
+        sim.synchronize()
+        seq_t += time.perf_counter() - t0
+        seq_work += sim.cell_count() * seq_steps
+        sim.close()
+    sequential = seq_work / seq_t
+    log(f"bench: sequential plain solves {sequential / 1e9:.2f} G/s")
+    return {"boxes": len(specs), "cells": cells, "steps": steps, "site_updates_per_s": batched,
+            "sequential_site_updates_per_s": sequential, "speedup_vs_sequential": batched / sequential,
+            "tile": list(tile)}
+
+
+def served_stream(bs, n_jobs=48, rate_hz=40.0, seed=99):
+    """Poisson arrivals of sweep jobs (edge in {48..192} step 16, 500-2000
+    steps, parameters as the batched sweep) into the continuous-batching
+    server under a 288 GB - 16 GB budget; p50/p99 latency and aggregate
+    throughput against sequential plain solves of the same jobs."""
+    rng_u = bs.pcg64mcg_f32(seed, 8 * n_jobs)
+    jobs = []
+    t = 0.0
+    for k in range(n_jobs):
+        u = [float(v) for v in rng_u[8 * k:8 * k + 8]]
+        edge = 48 + 16 * min(int(u[0] * 10), 9)
+        steps = 500 + int(u[1] * 1501)
+        eta = math.exp(math.log(0.02) + u[2] * (0.0 - math.log(0.02)))
+        model = bs.Model(eta=eta, seed_factor=0.7 + 0.8 * u[3], eps_ratio=min(0.6 + 0.4 * u[4], 0.999))
+        spec = bs.make_spec(grid=edge, model=model, nucleation=bs.NUC_UNIFORM_STEPS, bubbles=1 + min(int(u[5] * 4), 3),
+                            seed=k + 1, uniform_max_step=200, time_seed=5000 + k)
+        t += -math.log(max(1.0 - u[6], 1e-12)) / rate_hz
+        jobs.append((t, spec, steps))
+    budget = 288 * 10**9 - 16 * 10**9
+    srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20)
+    t0 = time.perf_counter()
+    ids = []
+    for (ta, spec, steps) in jobs:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        ids.append(srv.submit(spec, steps))
+    res = [srv.wait(j, 600) for j in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r.latency_s for r in res])
+    work = sum(spec.grid[0] * spec.grid[1] * spec.grid[2] * steps for (_, spec, steps) in jobs)
+    served = work / wall
+    # sequential plain solves of a sample of the same jobs, rate extrapolated
+    seq_t, seq_work = 0.0, 0
+    for (_, spec, steps) in jobs[:8]:
+        sim = bs.Simulation(spec)
+        n = min(steps, 200)
+        sim.advance(5)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        sim.advance(n)
+        sim.synchronize()
+        seq_t += time.perf_counter() - t1
+        seq_work += sim.cell_count() * n
+        sim.close()
+    sequential = seq_work / seq_t
+    log(f"bench: served {n_jobs} jobs at {rate_hz} Hz: wall {wall:.1f} s, {served / 1e9:.2f} G/s, "
+        f"p50 {np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s; sequential {sequential / 1e9:.2f} G/s")
+    return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "budget_bytes": int(st.budget_bytes),
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,
+            "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": all(r.status == 0 for r in res)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -112,6 +223,9 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    sim.close()
+    batch = None if args.no_batch else batched_sweep(bs)
+    served = None if args.no_serve else served_stream(bs)
     cpu = None if args.no_cpu else cpu_baseline(bs)
     out = {
         "metric": "RK3 site-updates/s at 512^3 f32 (fused stage kernel, % of HBM peak)",
@@ -133,9 +247,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
+        "batched_sweep": batch,
+        "served": served,
     }
     print(json.dumps(out))
-    sim.close()
     return 0
 
 
