@@ -96,10 +96,11 @@ int run_stage(BoxState &bx, int stage, int math, int kernel, TileShape tile, int
 }
 
 int default_zc(const uint32_t g[3], TileShape t) {
-    // aim for >= 2048 workgroups (8 per CU on 256 CUs) with chunks of >= 16
-    // planes, so the 4-plane prologue stays a small overhead
+    // aim for >= 16384 workgroups (64 per CU on 256 CUs: short z chunks
+    // balance the tail of the launch) with chunks of >= 16 planes, so the
+    // 4-plane prologue stays a small overhead
     const int64_t tiles = (int64_t)((g[0] + t.tx - 1) / t.tx) * ((g[1] + t.ty - 1) / t.ty);
-    int64_t chunks = (2048 + tiles - 1) / tiles;
+    int64_t chunks = (16384 + tiles - 1) / tiles;
     int zc = (int)((g[2] + chunks - 1) / chunks);
     if (zc < 16) zc = 16;
     if (zc > (int)g[2]) zc = (int)g[2];

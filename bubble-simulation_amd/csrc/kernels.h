@@ -209,44 +209,71 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         return sanitise<M>(eos_prim_from_cons<M>(fluid_in[i], f.x, P), P);
     };
 
-    // ---- prologue: column primitives for zb-2 .. zb+1, the face zb-1/2 ----
-    R ph_m, ph_0, ph_1, ph_tmp;
-    V4<R> qa = prim_at(cidx(P, gx, gy, wrapi(zb - 2, P.nz)), ph_tmp);
-    V4<R> qb = prim_at(cidx(P, gx, gy, wrapi(zb - 1, P.nz)), ph_m);
-    V4<R> qz = prim_at(cidx(P, gx, gy, wrapi(zb, P.nz)), ph_0);
-    V4<R> qz1 = prim_at(cidx(P, gx, gy, wrapi(zb + 1, P.nz)), ph_1);
-    const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
-    V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
-    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, ph_0, P);
-
+    // offsets within a z plane, fixed for the whole march
+    const int64_t plane = (int64_t)P.nx * P.ny;
+    const int64_t col = (int64_t)gy * P.nx + gx;
     // halo items of a plane: 4 TY x-halo cells, then 4 TX y-halo cells
     constexpr int NH = 4 * (TX + TY);
-
-    for (int z = zb; z < ze; ++z) {
-        const int zp2 = wrapi(z + 2, P.nz);
-        // (a) plane z+2 of this column (for the +z face)
-        R ph_2;
-        const V4<R> q2 = prim_at(cidx(P, gx, gy, zp2), ph_2);
-
-        // (b) plane z into LDS: interior from registers, halo recomputed
-        L.q[ty + 2][tx + 2] = qz;
-        L.phi[ty + 1][tx + 1] = ph_0;
-        for (int h = tid; h < NH; h += NT) {
+    constexpr int HPT = (NH + NT - 1) / NT;
+    int64_t hoff[HPT];
+    int hlds[HPT];  // (hy + 2) * (TX + 4) + (hx + 2), or -1
+    bool hphi[HPT];
+#pragma unroll
+    for (int k = 0; k < HPT; ++k) {
+        const int h = tid + k * NT;
+        hlds[k] = -1;
+        hoff[k] = 0;
+        hphi[k] = false;
+        if (h < NH) {
             int hx, hy;
             if (h < 4 * TY) {
                 const int r = h / 4, c = h % 4;
                 hy = r;
                 hx = c < 2 ? c - 2 : TX + (c - 2);
             } else {
-                const int k = h - 4 * TY;
-                const int r = k / TX, c = k % TX;
+                const int k2 = h - 4 * TY;
+                const int r = k2 / TX, c = k2 % TX;
                 hx = c;
                 hy = r < 2 ? r - 2 : TY + (r - 2);
             }
-            R phh;
-            const V4<R> qh = prim_at(cidx(P, wrapi(x0 + hx, P.nx), wrapi(y0 + hy, P.ny), z), phh);
-            L.q[hy + 2][hx + 2] = qh;
-            if (hx >= -1 && hx <= TX && hy >= -1 && hy <= TY) L.phi[hy + 1][hx + 1] = phh;
+            hoff[k] = (int64_t)wrapi(y0 + hy, P.ny) * P.nx + wrapi(x0 + hx, P.nx);
+            hlds[k] = (hy + 2) * (TX + 4) + (hx + 2);
+            hphi[k] = hx >= -1 && hx <= TX && hy >= -1 && hy <= TY;
+        }
+    }
+
+    // ---- prologue: column primitives for zb-2 .. zb+1, the face zb-1/2 ----
+    R ph_m, ph_0, ph_1, ph_tmp;
+    V4<R> qa = prim_at(wrapi(zb - 2, P.nz) * plane + col, ph_tmp);
+    V4<R> qb = prim_at(wrapi(zb - 1, P.nz) * plane + col, ph_m);
+    V4<R> qz = prim_at(wrapi(zb, P.nz) * plane + col, ph_0);
+    V4<R> qz1 = prim_at(wrapi(zb + 1, P.nz) * plane + col, ph_1);
+    const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
+    V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
+    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, ph_0, P);
+
+    V4<R> *qflat = &L.q[0][0];
+    for (int z = zb; z < ze; ++z) {
+        const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
+        // (a) plane z+2 of this column (for the +z face)
+        R ph_2;
+        const V4<R> q2 = prim_at(zp2 * plane + col, ph_2);
+
+        // (b) plane z into LDS: interior from registers, halo recomputed
+        L.q[ty + 2][tx + 2] = qz;
+        L.phi[ty + 1][tx + 1] = ph_0;
+        const int64_t zoff = (int64_t)z * plane;
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (hlds[k] >= 0) {
+                R phh;
+                const V4<R> qh = prim_at(zoff + hoff[k], phh);
+                qflat[hlds[k]] = qh;
+                if (hphi[k]) {
+                    const int hy = hlds[k] / (TX + 4) - 2, hx = hlds[k] % (TX + 4) - 2;
+                    L.phi[hy + 1][hx + 1] = phh;
+                }
+            }
         }
         __syncthreads();
 
@@ -275,7 +302,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 
         // (d) divergence, sources, RK combination
         if (valid) {
-            const int64_t i0 = cidx(P, gx, gy, z);
+            const int64_t i0 = zoff + col;
             R lap_sum = R(0);
             lap_sum = lap_sum + (pxm + pxp - R(2) * ph_0);
             const R gxv = R(0.5) * (pxp - pxm) * P.inv_dx;

@@ -85,6 +85,9 @@ static hipError_t fused_dispatch(const Box<R> &box, int stage, TileShape t, int 
     case 1: return fused_one<64, 8, M, R>(box, stage, zc, st);
     case 2: return fused_one<32, 8, M, R>(box, stage, zc, st);
     case 3: return fused_one<16, 16, M, R>(box, stage, zc, st);
+    case 4: return fused_one<64, 4, M, R>(box, stage, zc, st);
+    case 5: return fused_one<16, 8, M, R>(box, stage, zc, st);
+    case 6: return fused_one<32, 4, M, R>(box, stage, zc, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -96,6 +99,9 @@ static hipError_t fused_multi_dispatch(const Box<R> *dboxes, const BoxEvents *de
     case 1: return fused_multi_one<64, 8, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     case 2: return fused_multi_one<32, 8, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     case 3: return fused_multi_one<16, 16, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
+    case 4: return fused_multi_one<64, 4, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
+    case 5: return fused_multi_one<16, 8, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
+    case 6: return fused_multi_one<32, 4, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     default: return hipErrorInvalidValue;
     }
 }

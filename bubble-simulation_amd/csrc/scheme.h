@@ -64,6 +64,10 @@ template <class R> struct ExactMath {
         if (sa != sign(b) || sa != sign(c)) return R(0);
         return sa * min(abs(a), min(abs(b), abs(c)));
     }
+    // limit_slope (common.wgsl:125-127)
+    __host__ __device__ static inline R limit_slope(R dm, R dp) {
+        return minmod3(R(0.5) * (dm + dp), R(2) * dm, R(2) * dp);
+    }
 };
 
 // Hardware reciprocal / square root (1 ulp each, so a*rcp(b) stays inside
@@ -81,6 +85,11 @@ struct FastMathF32 {
     __device__ static inline float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     __device__ static inline float minmod3(float a, float b, float c) {
         return __builtin_amdgcn_fmed3f(a, __builtin_amdgcn_fmed3f(b, c, 0.0f), 0.0f);
+    }
+    // MC limiter: minmod3(c, 2 dm, 2 dp) with med3(2 dm, 2 dp, 0) written as
+    // 2 med3(dm, dp, 0) -- exact, since scaling by 2 commutes with the median
+    __device__ static inline float limit_slope(float dm, float dp) {
+        return __builtin_amdgcn_fmed3f(0.5f * (dm + dp), 2.0f * __builtin_amdgcn_fmed3f(dm, dp, 0.0f), 0.0f);
     }
 };
 
@@ -145,9 +154,16 @@ template <class M, class R>
 __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params<R> &P) {
     const R e_lab = M::max(u.x, P.e_floor);
     R zx = u.y, zy = u.z, zz = u.w;
-    const R zlen = M::sqrt(dot3(zx, zy, zz, zx, zy, zz));
+    const R z2raw = dot3(zx, zy, zz, zx, zy, zz);
     const R zcap = P.z_max * e_lab;
-    if (zlen > zcap) {
+    bool clamp;
+    if constexpr (M::fast) {
+        clamp = z2raw > zcap * zcap;  // |Z| > zcap without the square root (zcap > 0)
+    } else {
+        clamp = M::sqrt(z2raw) > zcap;
+    }
+    if (clamp) {
+        const R zlen = M::sqrt(z2raw);
         const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
         zx = zx * f; zy = zy * f; zz = zz * f;
     }
@@ -229,7 +245,7 @@ __host__ __device__ inline V4<R> hll_flux(const V4<R> &ql, R phil, const V4<R> &
 
 // limit_slope / limit_slope4 (common.wgsl:125-136): monotonised central.
 template <class M, class R> __host__ __device__ inline R limit_slope(R dm, R dp) {
-    return M::minmod3(R(0.5) * (dm + dp), R(2) * dm, R(2) * dp);
+    return M::limit_slope(dm, dp);
 }
 template <class M, class R>
 __host__ __device__ inline V4<R> limit_slope4(const V4<R> &dm, const V4<R> &dp) {
