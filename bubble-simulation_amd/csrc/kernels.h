@@ -176,14 +176,26 @@ __global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__
 //  every face sees the same operands in the same order.
 // ---------------------------------------------------------------------------
 template <int TX, int TY, class R> struct FusedLds {
-    V4<R> q[TY + 4][TX + 4];
-    R phi[TY + 2][TX + 2];
-    V4<R> fx[TY][TX + 1];
-    V4<R> fy[TY + 1][TX];
+    V4<R> q[TY + 4][TX + 4];  // sanitised primitives of plane z, 2-cell halo
+    V4<R> sx[TY][TX + 2];     // x slope of tile column c in [-1, TX] at [row][c + 1]
+    V4<R> sy[TY + 2][TX];     // y slope of tile row r in [-1, TY] at [r + 1][col]
+    R phi[TY + 2][TX + 2];    // phi of plane z, 1-cell halo
+    V4<R> fx[TY][TX + 1];     // flux through the face left of tile column c
+    V4<R> fy[TY + 1][TX];     // flux through the face below tile row r
 };
 
 template <class R> __device__ inline int64_t cidx(const Params<R> &P, int x, int y, int z) {
     return ((int64_t)z * P.ny + y) * P.nx + x;
+}
+
+// Element `idx` of a plane that starts at `base`: the byte offset is formed
+// in 32 bits (a plane is well below 4 GiB), so the load can use the scalar
+// base + vector offset addressing mode.
+template <class T> __device__ __forceinline__ const T &plane_at(const T *base, unsigned idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ T &plane_at(T *base, unsigned idx) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T));
 }
 
 template <int TX, int TY, class M, class R>
@@ -203,27 +215,21 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     RK3c<R>::get(stage, a0, a1);
     const R dt = P.dt;
 
-    auto prim_at = [&](int64_t i, R &phi_out) -> V4<R> {
-        const V2<R> f = field_in[i];
-        phi_out = f.x;
-        return sanitise<M>(eos_prim_from_cons<M>(fluid_in[i], f.x, P), P);
-    };
-
     // offsets within a z plane, fixed for the whole march
     const int64_t plane = (int64_t)P.nx * P.ny;
-    const int64_t col = (int64_t)gy * P.nx + gx;
+    const unsigned col = (unsigned)(gy * P.nx + gx);
     // halo items of a plane: 4 TY x-halo cells, then 4 TX y-halo cells
     constexpr int NH = 4 * (TX + TY);
     constexpr int HPT = (NH + NT - 1) / NT;
-    int64_t hoff[HPT];
+    unsigned hoff[HPT];
     int hlds[HPT];  // (hy + 2) * (TX + 4) + (hx + 2), or -1
-    bool hphi[HPT];
+    int hphi[HPT];  // (hy + 1) * (TX + 2) + (hx + 1) for the 1-cell halo, or -1
 #pragma unroll
     for (int k = 0; k < HPT; ++k) {
         const int h = tid + k * NT;
         hlds[k] = -1;
+        hphi[k] = -1;
         hoff[k] = 0;
-        hphi[k] = false;
         if (h < NH) {
             int hx, hy;
             if (h < 4 * TY) {
@@ -236,73 +242,135 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
                 hx = c;
                 hy = r < 2 ? r - 2 : TY + (r - 2);
             }
-            hoff[k] = (int64_t)wrapi(y0 + hy, P.ny) * P.nx + wrapi(x0 + hx, P.nx);
+            hoff[k] = (unsigned)(wrapi(y0 + hy, P.ny) * P.nx + wrapi(x0 + hx, P.nx));
             hlds[k] = (hy + 2) * (TX + 4) + (hx + 2);
-            hphi[k] = hx >= -1 && hx <= TX && hy >= -1 && hy <= TY;
+            if (hx >= -1 && hx <= TX && hy >= -1 && hy <= TY) hphi[k] = (hy + 1) * (TX + 2) + (hx + 1);
         }
     }
+    // face items: x faces (TX+1) x TY, y faces TX x (TY+1), spread over the
+    // threads (the y list is rotated by one wave so the partial last rounds
+    // of the two lists fall on different waves)
+    constexpr int NXF = (TX + 1) * TY;
+    constexpr int NYF = TX * (TY + 1);
+    constexpr int XPT = (NXF + NT - 1) / NT;
+    constexpr int YPT = (NYF + NT - 1) / NT;
+    int xfy[XPT], xfx[XPT], yfy[YPT], yfx[YPT];
+#pragma unroll
+    for (int k = 0; k < XPT; ++k) {
+        const int j = tid + k * NT;
+        xfy[k] = j < NXF ? j / (TX + 1) : -1;
+        xfx[k] = j < NXF ? j % (TX + 1) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < YPT; ++k) {
+        const int j = (tid + 64) % NT + k * NT;
+        yfy[k] = j < NYF ? j / TX : -1;
+        yfx[k] = j < NYF ? j % TX : 0;
+    }
 
-    // ---- prologue: column primitives for zb-2 .. zb+1, the face zb-1/2 ----
-    R ph_m, ph_0, ph_1, ph_tmp;
-    V4<R> qa = prim_at(wrapi(zb - 2, P.nz) * plane + col, ph_tmp);
-    V4<R> qb = prim_at(wrapi(zb - 1, P.nz) * plane + col, ph_m);
-    V4<R> qz = prim_at(wrapi(zb, P.nz) * plane + col, ph_0);
-    V4<R> qz1 = prim_at(wrapi(zb + 1, P.nz) * plane + col, ph_1);
+    auto prim_of = [&](const V2<R> &f, const V4<R> &u) -> V4<R> {
+        return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
+    };
+
+    // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
+    V2<R> f0, f1, ftmp;  // raw U_in of planes z, z+1 (phi, pi)
+    V4<R> u0, u1, utmp;  // raw U_in of planes z, z+1 (E, Z)
+    R ph_m;
+    ftmp = plane_at(field_in + wrapi(zb - 2, P.nz) * plane, col);
+    utmp = plane_at(fluid_in + wrapi(zb - 2, P.nz) * plane, col);
+    const V4<R> qa = prim_of(ftmp, utmp);
+    ftmp = plane_at(field_in + wrapi(zb - 1, P.nz) * plane, col);
+    utmp = plane_at(fluid_in + wrapi(zb - 1, P.nz) * plane, col);
+    const V4<R> qb = prim_of(ftmp, utmp);
+    ph_m = ftmp.x;
+    f0 = plane_at(field_in + wrapi(zb, P.nz) * plane, col);
+    u0 = plane_at(fluid_in + wrapi(zb, P.nz) * plane, col);
+    V4<R> qz = prim_of(f0, u0);
+    f1 = plane_at(field_in + wrapi(zb + 1, P.nz) * plane, col);
+    u1 = plane_at(fluid_in + wrapi(zb + 1, P.nz) * plane, col);
+    V4<R> qz1 = prim_of(f1, u1);
     const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
     V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
-    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, ph_0, P);
+    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, f0.x, P);
 
     V4<R> *qflat = &L.q[0][0];
+    R *phiflat = &L.phi[0][0];
     for (int z = zb; z < ze; ++z) {
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
+        const int64_t zoff = (int64_t)z * plane;
         // (a) plane z+2 of this column (for the +z face)
-        R ph_2;
-        const V4<R> q2 = prim_at(zp2 * plane + col, ph_2);
+        const V2<R> f2 = plane_at(field_in + zp2 * plane, col);
+        const V4<R> u2 = plane_at(fluid_in + zp2 * plane, col);
+        const V4<R> q2 = prim_of(f2, u2);
 
         // (b) plane z into LDS: interior from registers, halo recomputed
         L.q[ty + 2][tx + 2] = qz;
-        L.phi[ty + 1][tx + 1] = ph_0;
-        const int64_t zoff = (int64_t)z * plane;
+        L.phi[ty + 1][tx + 1] = f0.x;
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                R phh;
-                const V4<R> qh = prim_at(zoff + hoff[k], phh);
-                qflat[hlds[k]] = qh;
-                if (hphi[k]) {
-                    const int hy = hlds[k] / (TX + 4) - 2, hx = hlds[k] % (TX + 4) - 2;
-                    L.phi[hy + 1][hx + 1] = phh;
-                }
+                const V2<R> fh = plane_at(field_in + zoff, hoff[k]);
+                const V4<R> uh = plane_at(fluid_in + zoff, hoff[k]);
+                qflat[hlds[k]] = prim_of(fh, uh);
+                if (hphi[k] >= 0) phiflat[hphi[k]] = fh.x;
             }
         }
         __syncthreads();
 
-        // (c) x faces: fx[fy][f] between tile columns f-1 and f
-        constexpr int NXF = (TX + 1) * TY;
-        for (int j = tid; j < NXF; j += NT) {
-            const int fyi = j / (TX + 1), fxi = j % (TX + 1);
-            L.fx[fyi][fxi] = face_flux<0, M>(L.q[fyi + 2][fxi], L.q[fyi + 2][fxi + 1], L.q[fyi + 2][fxi + 2],
-                                             L.q[fyi + 2][fxi + 3], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1], P);
+        // (c) limited slopes, once per cell and axis (common.wgsl:125-136):
+        // each thread its own cell, then the columns / rows just outside
+        {
+            const V4<R> qc = L.q[ty + 2][tx + 2];
+            L.sx[ty][tx + 1] = limit_slope4<M>(sub4(qc, L.q[ty + 2][tx + 1]), sub4(L.q[ty + 2][tx + 3], qc));
+            L.sy[ty + 1][tx] = limit_slope4<M>(sub4(qc, L.q[ty + 1][tx + 2]), sub4(L.q[ty + 3][tx + 2], qc));
+        }
+        constexpr int NBS = 2 * (TX + TY);
+        for (int j = tid; j < NBS; j += NT) {
+            if (j < 2 * TY) {
+                const int r = j / 2, c = (j % 2) ? TX : -1;
+                const V4<R> qc = L.q[r + 2][c + 2];
+                L.sx[r][c + 1] = limit_slope4<M>(sub4(qc, L.q[r + 2][c + 1]), sub4(L.q[r + 2][c + 3], qc));
+            } else {
+                const int k = j - 2 * TY;
+                const int c = k / 2, r = (k % 2) ? TY : -1;
+                const V4<R> qc = L.q[r + 2][c + 2];
+                L.sy[r + 1][c] = limit_slope4<M>(sub4(qc, L.q[r + 1][c + 2]), sub4(L.q[r + 3][c + 2], qc));
+            }
+        }
+        __syncthreads();
+
+        // (d) x faces: fx[fy][f] between tile columns f-1 and f
+#pragma unroll
+        for (int k = 0; k < XPT; ++k) {
+            if (xfy[k] >= 0) {
+                const int fyi = xfy[k], fxi = xfx[k];
+                L.fx[fyi][fxi] = face_flux_sl<0, M>(L.q[fyi + 2][fxi + 1], L.sx[fyi][fxi], L.q[fyi + 2][fxi + 2],
+                                                    L.sx[fyi][fxi + 1], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1],
+                                                    P);
+            }
         }
         // y faces: fy[f][fx] between tile rows f-1 and f
-        constexpr int NYF = TX * (TY + 1);
-        for (int j = (tid + 64) % NT; j < NYF; j += NT) {
-            const int fyi = j / TX, fxi = j % TX;
-            L.fy[fyi][fxi] = face_flux<1, M>(L.q[fyi][fxi + 2], L.q[fyi + 1][fxi + 2], L.q[fyi + 2][fxi + 2],
-                                             L.q[fyi + 3][fxi + 2], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1], P);
+#pragma unroll
+        for (int k = 0; k < YPT; ++k) {
+            if (yfy[k] >= 0) {
+                const int fyi = yfy[k], fxi = yfx[k];
+                L.fy[fyi][fxi] = face_flux_sl<1, M>(L.q[fyi + 1][fxi + 2], L.sy[fyi][fxi], L.q[fyi + 2][fxi + 2],
+                                                    L.sy[fyi + 1][fxi], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1],
+                                                    P);
+            }
         }
         // own +z face
         const V4<R> sz1 = limit_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
-        const V4<R> fzp = face_flux_sl<2, M>(qz, sz, qz1, sz1, ph_0, ph_1, P);
+        const V4<R> fzp = face_flux_sl<2, M>(qz, sz, qz1, sz1, f0.x, f1.x, P);
 
         // field stencil from LDS / registers (step.wgsl:118-122)
         const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
         const R pym = L.phi[ty][tx + 1], pyp = L.phi[ty + 2][tx + 1];
         __syncthreads();
 
-        // (d) divergence, sources, RK combination
+        // (e) divergence, sources, RK combination
         if (valid) {
-            const int64_t i0 = zoff + col;
+            const R ph_0 = f0.x, ph_1 = f1.x;
             R lap_sum = R(0);
             lap_sum = lap_sum + (pxm + pxp - R(2) * ph_0);
             const R gxv = R(0.5) * (pxp - pxm) * P.inv_dx;
@@ -325,27 +393,27 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
             fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
             fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
-            const V2<R> fin = field_in[i0];
-            const V4<R> uin = fluid_in[i0];
             R phn, pin;
             V4<R> un;
             if (stage == 0) {
-                phn = fin.x; pin = fin.y; un = uin;   // U_in is U^n at stage 0 (same slot)
+                phn = f0.x; pin = f0.y; un = u0;   // U_in is U^n at stage 0 (same slot)
             } else {
-                const V2<R> fnn = field_n[i0];
-                phn = fnn.x; pin = fnn.y; un = fluid_n[i0];
+                const V2<R> fnn = plane_at(field_n + zoff, col);
+                phn = fnn.x; pin = fnn.y; un = plane_at(fluid_n + zoff, col);
             }
             R phi_o, pi_o;
             V4<R> fo;
-            stage_update<M>(ph_0, fin.y, qz, lap_sum, gxv, gyv, gzv, fd, uin, phn, pin, un, a0, a1, dt, P,
+            stage_update<M>(ph_0, f0.y, qz, lap_sum, gxv, gyv, gzv, fd, u0, phn, pin, un, a0, a1, dt, P,
                             phi_o, pi_o, fo);
-            field_out[i0] = V2<R>{phi_o, pi_o};
-            fluid_out[i0] = fo;
+            plane_at(field_out + zoff, col) = V2<R>{phi_o, pi_o};
+            plane_at(fluid_out + zoff, col) = fo;
         }
 
-        // (e) rotate the column ring
+        // (f) rotate the column ring
         qz = qz1; qz1 = q2; sz = sz1; fzm = fzp;
-        ph_m = ph_0; ph_0 = ph_1; ph_1 = ph_2;
+        ph_m = f0.x;
+        f0 = f1; u0 = u1;
+        f1 = f2; u1 = u2;
     }
 }
 

@@ -222,6 +222,72 @@ __host__ __device__ inline V4<R> sanitise(const V4<R> &q, const Params<R> &P) {
     return V4<R>{vx, vy, vz, p};
 }
 
+// sanitise that also returns |v|^2 of its result: the dot product it already
+// formed when nothing is clamped (the value a recomputation gives), else the
+// recomputed one (ExactMath) or vmax^2 itself (FastMath).
+template <class M, class R>
+__host__ __device__ inline V4<R> sanitise_v2(const V4<R> &q, const Params<R> &P, R &v2out) {
+    const R p = M::max(q.w, P.p_floor);
+    R vx = q.x, vy = q.y, vz = q.z;
+    const R v2 = dot3(vx, vy, vz, vx, vy, vz);
+    const R vmax2 = P.z_max * P.z_max;
+    v2out = v2;
+    if (v2 > vmax2) {
+        const R f = M::sqrt(M::div(vmax2, M::max(v2, Consts<R>::tiny)));
+        vx = vx * f; vy = vy * f; vz = vz * f;
+        if constexpr (M::fast) v2out = vmax2;
+        else v2out = dot3(vx, vy, vz, vx, vy, vz);
+    }
+    return V4<R>{vx, vy, vz, p};
+}
+
+// eos_cons_from_prim with |v|^2 of q given
+template <class M, class R>
+__host__ __device__ inline V4<R> eos_cons_from_prim_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P) {
+    const R v2 = M::min(v2raw, P.z_max * P.z_max);
+    const R w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
+    return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
+}
+
+// eos_wave_speeds with |v|^2 of q given
+template <int A, class M, class R>
+__host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P, R &lm,
+                                                   R &lp) {
+    const R v2 = M::min(v2raw, P.z_max * P.z_max);
+    const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
+    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs = M::sqrt(cs2);
+    const R denom = R(1) - v2 * cs2;
+    const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
+    const R a = M::div(vd * (R(1) - cs2), denom);
+    const R b = M::div(cs * M::sqrt(rad), denom);
+    lm = a - b;
+    lp = a + b;
+}
+
+// hll_flux (step.wgsl:77-90) along axis A, with |v|^2 of both states given
+// (the reference recomputes dot(v, v) in eos_cons_from_prim and
+// eos_wave_speeds; the value is the same).
+template <int A, class M, class R>
+__host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
+                                             const Params<R> &P) {
+    const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P);
+    const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P);
+    const V4<R> fl = eos_flux<A>(ql, ul);
+    const V4<R> fr = eos_flux<A>(qr, ur);
+    R llm, llp, lrm, lrp;
+    eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
+    eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
+    const R s_min = M::min(M::min(llm, lrm), R(0));
+    const R s_max = M::max(M::max(llp, lrp), R(0));
+    const R den = M::max(s_max - s_min, Consts<R>::den_floor);
+    const R smm = s_min * s_max;
+    return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
+                 M::div(s_max * fl.y - s_min * fr.y + smm * (ur.y - ul.y), den),
+                 M::div(s_max * fl.z - s_min * fr.z + smm * (ur.z - ul.z), den),
+                 M::div(s_max * fl.w - s_min * fr.w + smm * (ur.w - ul.w), den)};
+}
+
 // hll_flux (step.wgsl:77-90) along axis A.
 template <int A, class M, class R>
 __host__ __device__ inline V4<R> hll_flux(const V4<R> &ql, R phil, const V4<R> &qr, R phir,
@@ -272,15 +338,19 @@ __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, co
                                            const V4<R> &qrp, R phil, R phir, const Params<R> &P) {
     const V4<R> sl = limit_slope4<M>(sub4(ql, qlm), sub4(qr, ql));
     const V4<R> sr = limit_slope4<M>(sub4(qr, ql), sub4(qrp, qr));
-    return hll_flux<A, M>(sanitise<M>(half_plus(ql, sl), P), phil,
-                          sanitise<M>(half_minus(qr, sr), P), phir, P);
+    R v2l, v2r;
+    const V4<R> a = sanitise_v2<M>(half_plus(ql, sl), P, v2l);
+    const V4<R> b = sanitise_v2<M>(half_minus(qr, sr), P, v2r);
+    return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
 }
 // Same, with the two slopes already known.
 template <int A, class M, class R>
 __host__ __device__ inline V4<R> face_flux_sl(const V4<R> &ql, const V4<R> &sl, const V4<R> &qr,
                                               const V4<R> &sr, R phil, R phir, const Params<R> &P) {
-    return hll_flux<A, M>(sanitise<M>(half_plus(ql, sl), P), phil,
-                          sanitise<M>(half_minus(qr, sr), P), phir, P);
+    R v2l, v2r;
+    const V4<R> a = sanitise_v2<M>(half_plus(ql, sl), P, v2l);
+    const V4<R> b = sanitise_v2<M>(half_minus(qr, sr), P, v2r);
+    return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
 }
 
 // ---------------------------------------------------------------------------

@@ -137,3 +137,40 @@ def test_fused_equals_two_pass_fast_small_tiles(bs, gpu_available):
         res.append(sim.download())
     for k in range(2):
         assert _rel_l2(res[0][k], res[1][k]) < 1e-6
+
+
+TILES = ["32x16", "64x8", "32x8", "16x16", "64x4", "16x8", "32x4"]
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("zc", ["16", "7"])
+def test_every_tile_shape_matches_oracle(bs, orc, gpu_available, monkeypatch, tile, zc):
+    # the fused kernel's tiling (x/y tile, z chunk) changes which thread and
+    # which launch evaluates a face, never its operands: bitwise under ExactMath
+    monkeypatch.setenv("BS_TILE", tile)
+    monkeypatch.setenv("BS_ZC", zc)
+    grid = (36, 20, 18)
+    _log(f"case tile={tile} zc={zc}")
+    sched = _sched(bs, grid, [(0.0, 18.0, 10.0, 9.0), (0.4, 2.0, 17.0, 1.0)])
+    sim, o = _setup(bs, orc, grid, 0, bs.MATH_EXACT, bs.KERNEL_FUSED, sched)
+    sim.advance(4)
+    o.advance(4)
+    gf, gu = sim.download()
+    of, ou = o.state()
+    assert np.array_equal(gf, of) and np.array_equal(gu, ou)
+
+
+@pytest.mark.parametrize("grid", [(8, 8, 8), (8, 9, 10), (17, 8, 33)])
+def test_smallest_and_odd_boxes_match_oracle(bs, orc, gpu_available, grid):
+    # boxes smaller than a tile wrap the tile onto itself (min edge 8, sim.rs:47)
+    _log(f"case grid={grid}")
+    sched = _sched(bs, grid, [(0.0, grid[0] / 2, grid[1] / 2, grid[2] / 2)])
+    model = bs.Model(seed_cells=3.0)
+    sched[:, 4] = model.seed_radius()
+    for precision in (0, 1):
+        sim, o = _setup(bs, orc, grid, precision, bs.MATH_EXACT, bs.KERNEL_FUSED, sched, model=model)
+        sim.advance(5)
+        o.advance(5)
+        gf, gu = sim.download()
+        of, ou = o.state()
+        assert np.array_equal(gf, of) and np.array_equal(gu, ou)

@@ -97,6 +97,6 @@ def test_seed_threshold_table(bs, gpu_available, row):
         sim.close()
         print(f"eps_ratio {row['eps_ratio']} factor {factor}: broken {got:.2f}% (published {want})", flush=True)
         if want is None:
-            assert got < 0.05
+            assert got < 0.05  # "collapsed"
         else:
-            assert got == pytest.approx(want, abs=0.5 + 0.03 * want)
+            assert f"{got:.1f}" == f"{want:.1f}"  # the published precision
