@@ -22,7 +22,8 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbubblesim.so")
+# BUBBLESIM_LIB selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("BUBBLESIM_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libbubblesim.so")
 CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "bubble-sim")
 
 BS_OK, BS_ERR_INVALID, BS_ERR_OOM, BS_ERR_DEVICE, BS_ERR_NONFINITE, BS_ERR_NOT_READY, BS_ERR_NOT_FOUND = range(7)

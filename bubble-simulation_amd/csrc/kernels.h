@@ -14,6 +14,7 @@
 
 #include "scheme.h"
 
+
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
@@ -177,8 +178,6 @@ __global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__
 // ---------------------------------------------------------------------------
 template <int TX, int TY, class R> struct FusedLds {
     V4<R> q[TY + 4][TX + 4];  // sanitised primitives of plane z, 2-cell halo
-    V4<R> sx[TY][TX + 2];     // x slope of tile column c in [-1, TX] at [row][c + 1]
-    V4<R> sy[TY + 2][TX];     // y slope of tile row r in [-1, TY] at [r + 1][col]
     R phi[TY + 2][TX + 2];    // phi of plane z, 1-cell halo
     V4<R> fx[TY][TX + 1];     // flux through the face left of tile column c
     V4<R> fy[TY + 1][TX];     // flux through the face below tile row r
@@ -249,32 +248,24 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     }
     // face items: x faces (TX+1) x TY, y faces TX x (TY+1), spread over the
     // threads (the y list is rotated by one wave so the partial last rounds
-    // of the two lists fall on different waves)
+    // of the two lists fall on different waves); each face forms the two
+    // limited slopes it needs from four cells, which costs fewer cycles than
+    // a separate slope pass with its extra barrier
     constexpr int NXF = (TX + 1) * TY;
     constexpr int NYF = TX * (TY + 1);
     constexpr int XPT = (NXF + NT - 1) / NT;
     constexpr int YPT = (NYF + NT - 1) / NT;
-    int xfy[XPT], xfx[XPT], yfy[YPT], yfx[YPT];
-#pragma unroll
-    for (int k = 0; k < XPT; ++k) {
-        const int j = tid + k * NT;
-        xfy[k] = j < NXF ? j / (TX + 1) : -1;
-        xfx[k] = j < NXF ? j % (TX + 1) : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < YPT; ++k) {
-        const int j = (tid + 64) % NT + k * NT;
-        yfy[k] = j < NYF ? j / TX : -1;
-        yfx[k] = j < NYF ? j % TX : 0;
-    }
 
     auto prim_of = [&](const V2<R> &f, const V4<R> &u) -> V4<R> {
         return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
     };
 
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
-    V2<R> f0, f1, ftmp;  // raw U_in of planes z, z+1 (phi, pi)
-    V4<R> u0, u1, utmp;  // raw U_in of planes z, z+1 (E, Z)
+    // (phi of planes z-1, z, z+1 and the primitives of z, z+1 stay in
+    // registers; U_in of plane z is re-read at the update, an L2 hit, which
+    // keeps the kernel at 5 waves per SIMD)
+    V2<R> f0, f1, ftmp;
+    V4<R> u0, u1, utmp;
     R ph_m;
     ftmp = plane_at(field_in + wrapi(zb - 2, P.nz) * plane, col);
     utmp = plane_at(fluid_in + wrapi(zb - 2, P.nz) * plane, col);
@@ -317,46 +308,23 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         }
         __syncthreads();
 
-        // (c) limited slopes, once per cell and axis (common.wgsl:125-136):
-        // each thread its own cell, then the columns / rows just outside
-        {
-            const V4<R> qc = L.q[ty + 2][tx + 2];
-            L.sx[ty][tx + 1] = limit_slope4<M>(sub4(qc, L.q[ty + 2][tx + 1]), sub4(L.q[ty + 2][tx + 3], qc));
-            L.sy[ty + 1][tx] = limit_slope4<M>(sub4(qc, L.q[ty + 1][tx + 2]), sub4(L.q[ty + 3][tx + 2], qc));
-        }
-        constexpr int NBS = 2 * (TX + TY);
-        for (int j = tid; j < NBS; j += NT) {
-            if (j < 2 * TY) {
-                const int r = j / 2, c = (j % 2) ? TX : -1;
-                const V4<R> qc = L.q[r + 2][c + 2];
-                L.sx[r][c + 1] = limit_slope4<M>(sub4(qc, L.q[r + 2][c + 1]), sub4(L.q[r + 2][c + 3], qc));
-            } else {
-                const int k = j - 2 * TY;
-                const int c = k / 2, r = (k % 2) ? TY : -1;
-                const V4<R> qc = L.q[r + 2][c + 2];
-                L.sy[r + 1][c] = limit_slope4<M>(sub4(qc, L.q[r + 1][c + 2]), sub4(L.q[r + 3][c + 2], qc));
-            }
-        }
-        __syncthreads();
-
-        // (d) x faces: fx[fy][f] between tile columns f-1 and f
+        // (c) x and y faces, slopes formed inside each face from four cells
 #pragma unroll
         for (int k = 0; k < XPT; ++k) {
-            if (xfy[k] >= 0) {
-                const int fyi = xfy[k], fxi = xfx[k];
-                L.fx[fyi][fxi] = face_flux_sl<0, M>(L.q[fyi + 2][fxi + 1], L.sx[fyi][fxi], L.q[fyi + 2][fxi + 2],
-                                                    L.sx[fyi][fxi + 1], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1],
-                                                    P);
+            const int jj = tid + k * NT;
+            const int fyi = jj < NXF ? jj / (TX + 1) : -1, fxi = jj % (TX + 1);
+            if (fyi >= 0) {
+                L.fx[fyi][fxi] = face_flux<0, M>(L.q[fyi + 2][fxi], L.q[fyi + 2][fxi + 1], L.q[fyi + 2][fxi + 2],
+                                                 L.q[fyi + 2][fxi + 3], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1], P);
             }
         }
-        // y faces: fy[f][fx] between tile rows f-1 and f
 #pragma unroll
         for (int k = 0; k < YPT; ++k) {
-            if (yfy[k] >= 0) {
-                const int fyi = yfy[k], fxi = yfx[k];
-                L.fy[fyi][fxi] = face_flux_sl<1, M>(L.q[fyi + 1][fxi + 2], L.sy[fyi][fxi], L.q[fyi + 2][fxi + 2],
-                                                    L.sy[fyi + 1][fxi], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1],
-                                                    P);
+            const int jj = (tid + 64) % NT + k * NT;
+            const int fyi = jj < NYF ? jj / TX : -1, fxi = jj % TX;
+            if (fyi >= 0) {
+                L.fy[fyi][fxi] = face_flux<1, M>(L.q[fyi][fxi + 2], L.q[fyi + 1][fxi + 2], L.q[fyi + 2][fxi + 2],
+                                                 L.q[fyi + 3][fxi + 2], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1], P);
             }
         }
         // own +z face
@@ -368,7 +336,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         const R pym = L.phi[ty][tx + 1], pyp = L.phi[ty + 2][tx + 1];
         __syncthreads();
 
-        // (e) divergence, sources, RK combination
+        // (d) divergence, sources, RK combination
         if (valid) {
             const R ph_0 = f0.x, ph_1 = f1.x;
             R lap_sum = R(0);
@@ -393,36 +361,48 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
             fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
             fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
+            const R pi0 = plane_at(field_in + zoff, col).y;
+            const V4<R> uin = plane_at(fluid_in + zoff, col);
             R phn, pin;
             V4<R> un;
             if (stage == 0) {
-                phn = f0.x; pin = f0.y; un = u0;   // U_in is U^n at stage 0 (same slot)
+                phn = ph_0; pin = pi0; un = uin;   // U_in is U^n at stage 0 (same slot)
             } else {
                 const V2<R> fnn = plane_at(field_n + zoff, col);
                 phn = fnn.x; pin = fnn.y; un = plane_at(fluid_n + zoff, col);
             }
             R phi_o, pi_o;
             V4<R> fo;
-            stage_update<M>(ph_0, f0.y, qz, lap_sum, gxv, gyv, gzv, fd, u0, phn, pin, un, a0, a1, dt, P,
+            stage_update<M>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, phn, pin, un, a0, a1, dt, P,
                             phi_o, pi_o, fo);
             plane_at(field_out + zoff, col) = V2<R>{phi_o, pi_o};
             plane_at(fluid_out + zoff, col) = fo;
         }
 
-        // (f) rotate the column ring
+        // (e) rotate the column ring
         qz = qz1; qz1 = q2; sz = sz1; fzm = fzp;
         ph_m = f0.x;
-        f0 = f1; u0 = u1;
-        f1 = f2; u1 = u2;
+        f0.x = f1.x; f1.x = f2.x;
     }
 }
+
+// Minimum waves per SIMD the register allocator must leave room for: the
+// fast f32 stage fits 5 waves (<= 96 VGPRs) without spilling, and measures
+// faster there than at the 4 waves it would otherwise settle on; the exact
+// and f64 instantiations keep the compiler's choice.
+template <class M> struct Occupancy {
+    static constexpr int waves = 1;
+};
+template <> struct Occupancy<FastMathF32> {
+    static constexpr int waves = 5;
+};
 
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with
 // tile_x fastest; consecutive blocks land on different XCDs (round-robin
 // dispatch), so the logical tile of block b is remapped such that each XCD
 // works on a contiguous run of tiles and halo re-reads hit its own L2.
 template <int TX, int TY, class M, class R>
-__global__ void __launch_bounds__(TX *TY) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc, int nblocks) {
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc, int nblocks) {
     __shared__ FusedLds<TX, TY, R> L;
     int b = blockIdx.x;
     if ((nblocks & 7) == 0) b = (b & 7) * (nblocks >> 3) + (b >> 3);
@@ -457,7 +437,7 @@ struct BoxEvents {
     int pad;
 };
 template <int TX, int TY, class M, class R>
-__global__ void __launch_bounds__(TX *TY) k_stage_fused_multi(const Box<R> *__restrict__ boxes,
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused_multi(const Box<R> *__restrict__ boxes,
                                                               const BoxEvents *__restrict__ events,
                                                               const Task *__restrict__ tasks, int stage, int ntasks) {
     __shared__ FusedLds<TX, TY, R> L;
