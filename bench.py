@@ -120,7 +120,7 @@ def batched_sweep(bs, steps=200, seq_steps=40):
             "tile": list(tile)}
 
 
-def served_stream(bs, n_jobs=48, rate_hz=40.0, seed=99):
+def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     """Poisson arrivals of sweep jobs (edge in {48..192} step 16, 500-2000
     steps, parameters as the batched sweep) into the continuous-batching
     server under a 288 GB - 16 GB budget; p50/p99 latency and aggregate
@@ -138,7 +138,6 @@ def served_stream(bs, n_jobs=48, rate_hz=40.0, seed=99):
                             seed=k + 1, uniform_max_step=200, time_seed=5000 + k)
         t += -math.log(max(1.0 - u[6], 1e-12)) / rate_hz
         jobs.append((t, spec, steps))
-    budget = 288 * 10**9 - 16 * 10**9
     srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20)
     t0 = time.perf_counter()
     ids = []

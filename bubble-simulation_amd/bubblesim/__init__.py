@@ -102,7 +102,7 @@ class CServerConfig(C.Structure):
 
 
 class CJob(C.Structure):
-    _fields_ = [("spec", CSpec), ("steps", C.c_uint64), ("keep_state", C.c_int32), ("pad", C.c_uint32)]
+    _fields_ = [("spec", CSpec), ("steps", C.c_uint64), ("keep_state", C.c_int32), ("checksum", C.c_uint32)]
 
 
 class CJobResult(C.Structure):
@@ -623,8 +623,8 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, spec: CSpec, steps: int, keep_state: bool = False) -> int:
-        job = CJob(spec, steps, 1 if keep_state else 0, 0)
+    def submit(self, spec: CSpec, steps: int, keep_state: bool = False, checksum: bool = False) -> int:
+        job = CJob(spec, steps, 1 if keep_state else 0, 1 if checksum else 0)
         jid = C.c_uint64()
         _check(lib().bs_server_submit(self._h, C.byref(job), C.byref(jid)))
         return int(jid.value)

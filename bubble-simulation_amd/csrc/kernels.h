@@ -196,6 +196,55 @@ template <class T> __device__ __forceinline__ const T &plane_at(const T *base, u
 template <class T> __device__ __forceinline__ T &plane_at(T *base, unsigned idx) {
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T));
 }
+#ifndef BS_NT
+#define BS_NT 0
+#endif
+// streaming (non-temporal) access for data used once per stage: U^n and the
+// output slot, so they do not evict the planes the halo reads come back to
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <class R> struct VecT;
+template <> struct VecT<float> { typedef f4v v4; typedef f2v v2; };
+template <> struct VecT<double> { typedef d4v v4; typedef d2v v2; };
+
+template <class R> __device__ __forceinline__ V4<R> stream_load(const V4<R> *base, unsigned idx) {
+#if BS_NT
+    typedef typename VecT<R>::v4 T;
+    const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
+    return V4<R>{v.x, v.y, v.z, v.w};
+#else
+    return plane_at(base, idx);
+#endif
+}
+template <class R> __device__ __forceinline__ V2<R> stream_load(const V2<R> *base, unsigned idx) {
+#if BS_NT
+    typedef typename VecT<R>::v2 T;
+    const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
+    return V2<R>{v.x, v.y};
+#else
+    return plane_at(base, idx);
+#endif
+}
+template <class R> __device__ __forceinline__ void stream_store(V4<R> *base, unsigned idx, const V4<R> &v) {
+#if BS_NT
+    typedef typename VecT<R>::v4 T;
+    T t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
+#else
+    plane_at(base, idx) = v;
+#endif
+}
+template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, unsigned idx, const V2<R> &v) {
+#if BS_NT
+    typedef typename VecT<R>::v2 T;
+    T t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
+#else
+    plane_at(base, idx) = v;
+#endif
+}
 
 template <int TX, int TY, class M, class R>
 __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
@@ -368,15 +417,15 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             if (stage == 0) {
                 phn = ph_0; pin = pi0; un = uin;   // U_in is U^n at stage 0 (same slot)
             } else {
-                const V2<R> fnn = plane_at(field_n + zoff, col);
-                phn = fnn.x; pin = fnn.y; un = plane_at(fluid_n + zoff, col);
+                const V2<R> fnn = stream_load(field_n + zoff, col);
+                phn = fnn.x; pin = fnn.y; un = stream_load(fluid_n + zoff, col);
             }
             R phi_o, pi_o;
             V4<R> fo;
             stage_update<M>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, phn, pin, un, a0, a1, dt, P,
                             phi_o, pi_o, fo);
-            plane_at(field_out + zoff, col) = V2<R>{phi_o, pi_o};
-            plane_at(fluid_out + zoff, col) = fo;
+            stream_store(field_out + zoff, col, V2<R>{phi_o, pi_o});
+            stream_store(fluid_out + zoff, col, fo);
         }
 
         // (e) rotate the column ring

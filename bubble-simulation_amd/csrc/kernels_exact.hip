@@ -88,6 +88,7 @@ static hipError_t fused_dispatch(const Box<R> &box, int stage, TileShape t, int 
     case 4: return fused_one<64, 4, M, R>(box, stage, zc, st);
     case 5: return fused_one<16, 8, M, R>(box, stage, zc, st);
     case 6: return fused_one<32, 4, M, R>(box, stage, zc, st);
+    case 7: return fused_one<16, 15, M, R>(box, stage, zc, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -102,6 +103,7 @@ static hipError_t fused_multi_dispatch(const Box<R> *dboxes, const BoxEvents *de
     case 4: return fused_multi_one<64, 4, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     case 5: return fused_multi_one<16, 8, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     case 6: return fused_multi_one<32, 4, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
+    case 7: return fused_multi_one<16, 15, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -33,6 +33,7 @@ hipError_t launch_stage_fused_fast(const Box<float> &box, int stage, TileShape t
     case 4: return fused_one<64, 4>(box, stage, zc, st);
     case 5: return fused_one<16, 8>(box, stage, zc, st);
     case 6: return fused_one<32, 4>(box, stage, zc, st);
+    case 7: return fused_one<16, 15>(box, stage, zc, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -47,6 +48,7 @@ hipError_t launch_stage_fused_multi_fast(const Box<float> *dboxes, const BoxEven
     case 4: return fused_multi_one<64, 4>(dboxes, devents, dtasks, ntasks, stage, st);
     case 5: return fused_multi_one<16, 8>(dboxes, devents, dtasks, ntasks, stage, st);
     case 6: return fused_multi_one<32, 4>(dboxes, devents, dtasks, ntasks, stage, st);
+    case 7: return fused_multi_one<16, 15>(dboxes, devents, dtasks, ntasks, stage, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -17,8 +17,9 @@ struct TileShape {
 };
 
 // Tile shapes the fused kernel is instantiated for (x width, y height).
-constexpr int kNumTiles = 7;
-constexpr TileShape kTiles[kNumTiles] = {{32, 16}, {64, 8}, {32, 8}, {16, 16}, {64, 4}, {16, 8}, {32, 4}};
+// (16, 15): 255 x-faces and 256 y-faces per plane fill whole waves of 64
+constexpr int kNumTiles = 8;
+constexpr TileShape kTiles[kNumTiles] = {{32, 16}, {64, 8}, {32, 8}, {16, 16}, {64, 4}, {16, 8}, {32, 4}, {16, 15}};
 int tile_index(TileShape t);  // -1 if not instantiated
 
 template <class R> hipError_t launch_init(V2<R> *field, V4<R> *fluid, int64_t n, R e_ref, hipStream_t st);

@@ -84,8 +84,10 @@ static void finish_job(bs_server *s, Job &j, int status) {
         if (launch_diagnostics(e->bx, s->rs, s->stream) == BS_OK && hipStreamSynchronize(s->stream) == hipSuccess)
             diagnostics_from_result(*s->rs.hres, e->bx.n_cells, &d.res.final_diag);
         const size_t r = real_size(e->spec.precision);
-        std::vector<unsigned char> buf(e->bx.n_cells * 6 * r);
-        if (download_state(e->bx, s->stream, buf.data(), buf.data() + e->bx.n_cells * 2 * r) == BS_OK) {
+        std::vector<unsigned char> buf;
+        if (j.job.keep_state || j.job.checksum) buf.resize(e->bx.n_cells * 6 * r);
+        if (!buf.empty() &&
+            download_state(e->bx, s->stream, buf.data(), buf.data() + e->bx.n_cells * 2 * r) == BS_OK) {
             uint64_t h = 1469598103934665603ULL;
             for (unsigned char c : buf) {
                 h ^= c;

@@ -384,7 +384,7 @@ typedef struct bs_job {
     bs_spec spec;
     uint64_t steps;              /* steps to advance                            */
     int32_t keep_state;          /* 1: keep the final state for bs_server_fetch */
-    uint32_t pad;
+    uint32_t checksum;           /* 1: fill state_checksum (reads the state back) */
 } bs_job;
 
 typedef struct bs_job_result {
@@ -395,7 +395,8 @@ typedef struct bs_job_result {
     double admit_s;
     double finish_s;
     double latency_s;            /* finish - submit                            */
-    uint64_t state_checksum;     /* FNV-1a of the final field+fluid bytes      */
+    uint64_t state_checksum;     /* FNV-1a of the final field+fluid bytes, when
+                                    requested (keep_state or checksum), else 0  */
     int32_t status;              /* BS_OK or an error code                      */
     uint32_t pad;
 } bs_job_result;

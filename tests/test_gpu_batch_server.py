@@ -74,10 +74,10 @@ def test_server_alone_equals_mixed(bs, gpu_available):
     srv = bs.Server(budget_bytes=2 << 30, chunk_steps=5)
     alone = {}
     for k, (s, n) in enumerate(zip(specs, steps)):
-        r = srv.wait(srv.submit(s, n), 300)
+        r = srv.wait(srv.submit(s, n, checksum=True), 300)
         assert r.status == 0 and r.steps == n
         alone[k] = r.state_checksum
-    ids = [srv.submit(s, n) for s, n in zip(specs, steps)]
+    ids = [srv.submit(s, n, checksum=True) for s, n in zip(specs, steps)]
     res = _wait_all(srv, ids)
     for k, j in enumerate(ids):
         assert res[j].status == 0 and res[j].steps == steps[k]

@@ -139,7 +139,7 @@ def test_fused_equals_two_pass_fast_small_tiles(bs, gpu_available):
         assert _rel_l2(res[0][k], res[1][k]) < 1e-6
 
 
-TILES = ["32x16", "64x8", "32x8", "16x16", "64x4", "16x8", "32x4"]
+TILES = ["32x16", "64x8", "32x8", "16x16", "64x4", "16x8", "32x4", "16x15"]
 
 
 @pytest.mark.parametrize("tile", TILES)
