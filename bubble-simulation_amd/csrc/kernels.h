@@ -196,11 +196,9 @@ template <class T> __device__ __forceinline__ const T &plane_at(const T *base, u
 template <class T> __device__ __forceinline__ T &plane_at(T *base, unsigned idx) {
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T));
 }
-#ifndef BS_NT
-#define BS_NT 0
-#endif
-// streaming (non-temporal) access for data used once per stage: U^n and the
-// output slot, so they do not evict the planes the halo reads come back to
+// Streaming (non-temporal) access for data used once per stage -- U^n and the
+// output slot -- so they do not evict from L2 the U_in planes that the halo
+// reads and the update's re-read come back to (measured +3 % at 512^3).
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -210,40 +208,24 @@ template <> struct VecT<float> { typedef f4v v4; typedef f2v v2; };
 template <> struct VecT<double> { typedef d4v v4; typedef d2v v2; };
 
 template <class R> __device__ __forceinline__ V4<R> stream_load(const V4<R> *base, unsigned idx) {
-#if BS_NT
     typedef typename VecT<R>::v4 T;
     const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
     return V4<R>{v.x, v.y, v.z, v.w};
-#else
-    return plane_at(base, idx);
-#endif
 }
 template <class R> __device__ __forceinline__ V2<R> stream_load(const V2<R> *base, unsigned idx) {
-#if BS_NT
     typedef typename VecT<R>::v2 T;
     const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
     return V2<R>{v.x, v.y};
-#else
-    return plane_at(base, idx);
-#endif
 }
 template <class R> __device__ __forceinline__ void stream_store(V4<R> *base, unsigned idx, const V4<R> &v) {
-#if BS_NT
     typedef typename VecT<R>::v4 T;
     T t = {v.x, v.y, v.z, v.w};
     __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
-#else
-    plane_at(base, idx) = v;
-#endif
 }
 template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, unsigned idx, const V2<R> &v) {
-#if BS_NT
     typedef typename VecT<R>::v2 T;
     T t = {v.x, v.y};
     __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
-#else
-    plane_at(base, idx) = v;
-#endif
 }
 
 template <int TX, int TY, class M, class R>

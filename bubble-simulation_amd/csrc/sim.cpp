@@ -21,7 +21,7 @@ struct bs_sim {
     ReduceScratch rs;
     hipEvent_t diag_event = nullptr;
     bool diag_pending = false;
-    TileShape tile{16, 16};
+    TileShape tile{32, 8};
     int zc = 16;
     // per-stage timing (bs_sim_advance_timed)
     bool timing = false;
@@ -43,7 +43,7 @@ int sim_alloc(bs_sim *s) {
             return set_error(BS_ERR_OOM, "out of device memory allocating the primitive cache");
         }
     }
-    s->tile = tile_from_env(TileShape{16, 16});
+    s->tile = tile_from_env(TileShape{32, 8});
     s->zc = default_zc(s->spec.grid, s->tile);
     return BS_OK;
 }
