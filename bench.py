@@ -175,6 +175,29 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
             "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": all(r.status == 0 for r in res)}
 
 
+def copy_calibration(nbytes=1 << 30, repeats=3):
+    """Three plain device-to-device copies of a known size (hipMemcpy).
+
+    Run under rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE): the counter's
+    reading for the copy kernel against the nbytes it moves gives the factor
+    that converts the counter into HBM bytes for the stage kernel."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    size = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(a), size) or hip.hipMalloc(ctypes.byref(b), size):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, size)
+    for _ in range(repeats):
+        if hip.hipMemcpy(b, a, size, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {repeats} device copies of {nbytes} bytes")
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,7 +207,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 1 GiB device copies (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
