@@ -277,15 +277,26 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             if (hx >= -1 && hx <= TX && hy >= -1 && hy <= TY) hphi[k] = (hy + 1) * (TX + 2) + (hx + 1);
         }
     }
-    // face items: x faces (TX+1) x TY, y faces TX x (TY+1), spread over the
-    // threads (the y list is rotated by one wave so the partial last rounds
-    // of the two lists fall on different waves); each face forms the two
-    // limited slopes it needs from four cells, which costs fewer cycles than
-    // a separate slope pass with its extra barrier
-    constexpr int NXF = (TX + 1) * TY;
-    constexpr int NYF = TX * (TY + 1);
-    constexpr int XPT = (NXF + NT - 1) / NT;
-    constexpr int YPT = (NYF + NT - 1) / NT;
+    // faces: round 1 gives every thread the x face left of its cell, round 2
+    // the y face below it; the TY + TX faces on the far edges (x = TX, y = TY)
+    // form one more round in which the y faces are evaluated as x faces of
+    // the component-swapped states (vx <-> vy), so that round is one uniform
+    // code path instead of one per axis.  The swap commutes with every
+    // operation of the face (the sums of squares add the same two products
+    // in either order), so the result is the y face's, bit for bit.
+    constexpr int NTAIL = TX + TY;
+    const bool tail = tid < NTAIL;
+    const bool tail_y = tid >= TY;
+    int tq0 = 0, tqs = 0, tp0 = 0, tps = 0;
+    V4<R> *tout = nullptr;
+    if (tail) {
+        const int c = tail_y ? tid - TY : tid;
+        tq0 = tail_y ? TY * (TX + 4) + (c + 2) : (c + 2) * (TX + 4) + TX;
+        tqs = tail_y ? TX + 4 : 1;
+        tp0 = tail_y ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + TX;
+        tps = tail_y ? TX + 2 : 1;
+        tout = tail_y ? &L.fy[TY][c] : &L.fx[c][TX];
+    }
 
     auto prim_of = [&](const V2<R> &f, const V4<R> &u) -> V4<R> {
         return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
@@ -340,23 +351,18 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         __syncthreads();
 
         // (c) x and y faces, slopes formed inside each face from four cells
-#pragma unroll
-        for (int k = 0; k < XPT; ++k) {
-            const int jj = tid + k * NT;
-            const int fyi = jj < NXF ? jj / (TX + 1) : -1, fxi = jj % (TX + 1);
-            if (fyi >= 0) {
-                L.fx[fyi][fxi] = face_flux<0, M>(L.q[fyi + 2][fxi], L.q[fyi + 2][fxi + 1], L.q[fyi + 2][fxi + 2],
-                                                 L.q[fyi + 2][fxi + 3], L.phi[fyi + 1][fxi], L.phi[fyi + 1][fxi + 1], P);
+        L.fx[ty][tx] = face_flux<0, M>(L.q[ty + 2][tx], L.q[ty + 2][tx + 1], L.q[ty + 2][tx + 2], L.q[ty + 2][tx + 3],
+                                       L.phi[ty + 1][tx], L.phi[ty + 1][tx + 1], P);
+        L.fy[ty][tx] = face_flux<1, M>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], L.q[ty + 2][tx + 2], L.q[ty + 3][tx + 2],
+                                       L.phi[ty][tx + 1], L.phi[ty + 1][tx + 1], P);
+        if (tail) {
+            V4<R> c0 = qflat[tq0], c1 = qflat[tq0 + tqs], c2 = qflat[tq0 + 2 * tqs], c3 = qflat[tq0 + 3 * tqs];
+            if (tail_y) {
+                c0 = swap_xy(c0); c1 = swap_xy(c1); c2 = swap_xy(c2); c3 = swap_xy(c3);
             }
-        }
-#pragma unroll
-        for (int k = 0; k < YPT; ++k) {
-            const int jj = (tid + 64) % NT + k * NT;
-            const int fyi = jj < NYF ? jj / TX : -1, fxi = jj % TX;
-            if (fyi >= 0) {
-                L.fy[fyi][fxi] = face_flux<1, M>(L.q[fyi][fxi + 2], L.q[fyi + 1][fxi + 2], L.q[fyi + 2][fxi + 2],
-                                                 L.q[fyi + 3][fxi + 2], L.phi[fyi][fxi + 1], L.phi[fyi + 1][fxi + 1], P);
-            }
+            V4<R> ft = face_flux<0, M>(c0, c1, c2, c3, phiflat[tp0], phiflat[tp0 + tps], P);
+            if (tail_y) ft = swap_yz(ft);
+            *tout = ft;
         }
         // own +z face
         const V4<R> sz1 = limit_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));

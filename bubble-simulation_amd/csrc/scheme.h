@@ -321,6 +321,10 @@ __host__ __device__ inline V4<R> limit_slope4(const V4<R> &dm, const V4<R> &dp) 
 template <class R> __host__ __device__ inline V4<R> sub4(const V4<R> &a, const V4<R> &b) {
     return V4<R>{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
 }
+// exchange of the x and y velocity slots of a primitive state, and of the
+// x and y momentum slots of a flux (a y face evaluated as an x face)
+template <class R> __host__ __device__ inline V4<R> swap_xy(const V4<R> &a) { return V4<R>{a.y, a.x, a.z, a.w}; }
+template <class R> __host__ __device__ inline V4<R> swap_yz(const V4<R> &a) { return V4<R>{a.x, a.z, a.y, a.w}; }
 // q + 0.5 s  and  q - 0.5 s  (step.wgsl:136-143)
 template <class R> __host__ __device__ inline V4<R> half_plus(const V4<R> &q, const V4<R> &s) {
     return V4<R>{q.x + R(0.5) * s.x, q.y + R(0.5) * s.y, q.z + R(0.5) * s.z, q.w + R(0.5) * s.w};
