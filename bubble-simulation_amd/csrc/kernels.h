@@ -15,6 +15,7 @@
 #include "scheme.h"
 
 
+
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
@@ -187,18 +188,27 @@ template <class R> __device__ inline int64_t cidx(const Params<R> &P, int x, int
     return ((int64_t)z * P.ny + y) * P.nx + x;
 }
 
-// Element `idx` of a plane that starts at `base`: the byte offset is formed
-// in 32 bits (a plane is well below 4 GiB), so the load can use the scalar
-// base + vector offset addressing mode.
-template <class T> __device__ __forceinline__ const T &plane_at(const T *base, unsigned idx) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(T));
+// A pointer that is the same in every lane of the wave, moved into scalar
+// registers and tagged as global memory.  The compiler cannot see through
+// readfirstlane, so it keeps the z-march's plane bases as scalar values
+// instead of rewriting every access into a per-lane 64-bit pointer advanced
+// by a vector add each plane.
+template <class T> using global_t = __attribute__((address_space(1))) T;
+__device__ __forceinline__ global_t<char> *uniform_ptr(const void *p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return reinterpret_cast<global_t<char> *>(((uint64_t)hi << 32) | lo);
 }
-template <class T> __device__ __forceinline__ T &plane_at(T *base, unsigned idx) {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T));
-}
-// Streaming (non-temporal) access for data used once per stage -- U^n and the
-// output slot -- so they do not evict from L2 the U_in planes that the halo
-// reads and the update's re-read come back to (measured +3 % at 512^3).
+
+// Element `idx` of a plane that starts at `base` (the same in every lane):
+// the byte offset is formed in 32 bits (a plane is well below 4 GiB), so the
+// access uses the scalar base + vector offset addressing mode.  Accesses go
+// through the clang vector types, which load and store as one dwordx2 /
+// dwordx4; the streaming forms are non-temporal, for data used once per
+// stage -- U^n and the output slot -- so they do not evict from L2 the U_in
+// planes that the halo reads and the update's re-read come back to
+// (measured +3 % at 512^3).
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -207,25 +217,32 @@ template <class R> struct VecT;
 template <> struct VecT<float> { typedef f4v v4; typedef f2v v2; };
 template <> struct VecT<double> { typedef d4v v4; typedef d2v v2; };
 
+template <class T, class E> __device__ __forceinline__ global_t<T> *elem_ptr(const E *base, unsigned idx) {
+    return reinterpret_cast<global_t<T> *>(uniform_ptr(base) + idx * (unsigned)sizeof(E));
+}
+template <class R> __device__ __forceinline__ V4<R> plane_load(const V4<R> *base, unsigned idx) {
+    const typename VecT<R>::v4 v = *elem_ptr<const typename VecT<R>::v4>(base, idx);
+    return V4<R>{v.x, v.y, v.z, v.w};
+}
+template <class R> __device__ __forceinline__ V2<R> plane_load(const V2<R> *base, unsigned idx) {
+    const typename VecT<R>::v2 v = *elem_ptr<const typename VecT<R>::v2>(base, idx);
+    return V2<R>{v.x, v.y};
+}
 template <class R> __device__ __forceinline__ V4<R> stream_load(const V4<R> *base, unsigned idx) {
-    typedef typename VecT<R>::v4 T;
-    const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
+    const typename VecT<R>::v4 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v4>(base, idx));
     return V4<R>{v.x, v.y, v.z, v.w};
 }
 template <class R> __device__ __forceinline__ V2<R> stream_load(const V2<R> *base, unsigned idx) {
-    typedef typename VecT<R>::v2 T;
-    const T v = __builtin_nontemporal_load(reinterpret_cast<const T *>(&plane_at(base, idx)));
+    const typename VecT<R>::v2 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v2>(base, idx));
     return V2<R>{v.x, v.y};
 }
 template <class R> __device__ __forceinline__ void stream_store(V4<R> *base, unsigned idx, const V4<R> &v) {
-    typedef typename VecT<R>::v4 T;
-    T t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
+    typename VecT<R>::v4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v4>(base, idx));
 }
 template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, unsigned idx, const V2<R> &v) {
-    typedef typename VecT<R>::v2 T;
-    T t = {v.x, v.y};
-    __builtin_nontemporal_store(t, reinterpret_cast<T *>(&plane_at(base, idx)));
+    typename VecT<R>::v2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(base, idx));
 }
 
 template <int TX, int TY, class M, class R>
@@ -298,29 +315,33 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         tout = tail_y ? &L.fy[TY][c] : &L.fx[c][TX];
     }
 
+    // sanitised primitives of a cell.  eos_prim_from_cons already floors p
+    // and caps |Z| <= z_max E, so |v| = |Z| / (E + p) < z_max: sanitise is
+    // the identity up to rounding, and FastMath leaves it out.
     auto prim_of = [&](const V2<R> &f, const V4<R> &u) -> V4<R> {
-        return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
+        if constexpr (M::fast) return eos_prim_from_cons<M>(u, f.x, P);
+        else return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
     };
 
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
     // (phi of planes z-1, z, z+1 and the primitives of z, z+1 stay in
-    // registers; U_in of plane z is re-read at the update, an L2 hit, which
-    // keeps the kernel at 5 waves per SIMD)
+    // registers; U_in of plane z is loaded again for the update, an L2 hit,
+    // which keeps the kernel at 5 waves per SIMD)
     V2<R> f0, f1, ftmp;
     V4<R> u0, u1, utmp;
     R ph_m;
-    ftmp = plane_at(field_in + wrapi(zb - 2, P.nz) * plane, col);
-    utmp = plane_at(fluid_in + wrapi(zb - 2, P.nz) * plane, col);
+    ftmp = plane_load(field_in + wrapi(zb - 2, P.nz) * plane, col);
+    utmp = plane_load(fluid_in + wrapi(zb - 2, P.nz) * plane, col);
     const V4<R> qa = prim_of(ftmp, utmp);
-    ftmp = plane_at(field_in + wrapi(zb - 1, P.nz) * plane, col);
-    utmp = plane_at(fluid_in + wrapi(zb - 1, P.nz) * plane, col);
+    ftmp = plane_load(field_in + wrapi(zb - 1, P.nz) * plane, col);
+    utmp = plane_load(fluid_in + wrapi(zb - 1, P.nz) * plane, col);
     const V4<R> qb = prim_of(ftmp, utmp);
     ph_m = ftmp.x;
-    f0 = plane_at(field_in + wrapi(zb, P.nz) * plane, col);
-    u0 = plane_at(fluid_in + wrapi(zb, P.nz) * plane, col);
+    f0 = plane_load(field_in + wrapi(zb, P.nz) * plane, col);
+    u0 = plane_load(fluid_in + wrapi(zb, P.nz) * plane, col);
     V4<R> qz = prim_of(f0, u0);
-    f1 = plane_at(field_in + wrapi(zb + 1, P.nz) * plane, col);
-    u1 = plane_at(fluid_in + wrapi(zb + 1, P.nz) * plane, col);
+    f1 = plane_load(field_in + wrapi(zb + 1, P.nz) * plane, col);
+    u1 = plane_load(fluid_in + wrapi(zb + 1, P.nz) * plane, col);
     V4<R> qz1 = prim_of(f1, u1);
     const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
     V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
@@ -331,9 +352,19 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     for (int z = zb; z < ze; ++z) {
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
         const int64_t zoff = (int64_t)z * plane;
+        // the update's operands U_in(z) and U^n(z) of this cell, loaded first
+        // so that they arrive while the faces are computed
+        const R pi0 = plane_load(field_in + zoff, col).y;
+        const V4<R> uin = plane_load(fluid_in + zoff, col);
+        V2<R> fnn{};
+        V4<R> un{};
+        if (stage != 0) {
+            fnn = stream_load(field_n + zoff, col);
+            un = stream_load(fluid_n + zoff, col);
+        }
         // (a) plane z+2 of this column (for the +z face)
-        const V2<R> f2 = plane_at(field_in + zp2 * plane, col);
-        const V4<R> u2 = plane_at(fluid_in + zp2 * plane, col);
+        const V2<R> f2 = plane_load(field_in + zp2 * plane, col);
+        const V4<R> u2 = plane_load(fluid_in + zp2 * plane, col);
         const V4<R> q2 = prim_of(f2, u2);
 
         // (b) plane z into LDS: interior from registers, halo recomputed
@@ -342,8 +373,8 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                const V2<R> fh = plane_at(field_in + zoff, hoff[k]);
-                const V4<R> uh = plane_at(fluid_in + zoff, hoff[k]);
+                const V2<R> fh = plane_load(field_in + zoff, hoff[k]);
+                const V4<R> uh = plane_load(fluid_in + zoff, hoff[k]);
                 qflat[hlds[k]] = prim_of(fh, uh);
                 if (hphi[k] >= 0) phiflat[hphi[k]] = fh.x;
             }
@@ -398,15 +429,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
             fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
             fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
-            const R pi0 = plane_at(field_in + zoff, col).y;
-            const V4<R> uin = plane_at(fluid_in + zoff, col);
-            R phn, pin;
-            V4<R> un;
+            R phn = fnn.x, pin = fnn.y;
             if (stage == 0) {
                 phn = ph_0; pin = pi0; un = uin;   // U_in is U^n at stage 0 (same slot)
-            } else {
-                const V2<R> fnn = stream_load(field_n + zoff, col);
-                phn = fnn.x; pin = fnn.y; un = stream_load(fluid_n + zoff, col);
             }
             R phi_o, pi_o;
             V4<R> fo;
@@ -427,11 +452,14 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 // fast f32 stage fits 5 waves (<= 96 VGPRs) without spilling, and measures
 // faster there than at the 4 waves it would otherwise settle on; the exact
 // and f64 instantiations keep the compiler's choice.
+#ifndef BS_FAST_WAVES
+#define BS_FAST_WAVES 5
+#endif
 template <class M> struct Occupancy {
     static constexpr int waves = 1;
 };
 template <> struct Occupancy<FastMathF32> {
-    static constexpr int waves = 5;
+    static constexpr int waves = BS_FAST_WAVES;
 };
 
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with

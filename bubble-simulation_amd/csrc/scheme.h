@@ -55,6 +55,8 @@ template <class R> struct ExactMath {
     __host__ __device__ static inline R sqrt(R x) { return std::sqrt(x); }
     __host__ __device__ static inline R min(R a, R b) { return b < a ? b : a; }
     __host__ __device__ static inline R max(R a, R b) { return a < b ? b : a; }
+    __host__ __device__ static inline R min3(R a, R b, R c) { return min(min(a, b), c); }
+    __host__ __device__ static inline R max3(R a, R b, R c) { return max(max(a, b), c); }
     __host__ __device__ static inline R abs(R a) { return std::fabs(a); }
     __host__ __device__ static inline R inv_sqrt(R x) { return R(1) / std::sqrt(x); }
     __host__ __device__ static inline R sign(R a) { return a > R(0) ? R(1) : (a < R(0) ? R(-1) : R(0)); }
@@ -75,12 +77,20 @@ template <class R> struct ExactMath {
 // minmod written as a median: minmod(a, b) = med3(a, b, 0), so
 // minmod3(c, 2a, 2b) = med3(c, 2 med3(a, b, 0), 0) -- the same value as the
 // sign-test form for every non-NaN input (up to the sign of a zero result).
+// Two-operand min / max are medians against -FLT_MAX / +FLT_MAX (equal to
+// min / max for finite operands): v_med3 takes its operands as they are,
+// where v_min / v_max in IEEE mode first quiet every operand the compiler
+// cannot prove canonical (loaded values, kernel arguments, values merged
+// after a branch) with an extra instruction each.
 struct FastMathF32 {
     static constexpr bool fast = true;
     __device__ static inline float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static inline float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-    __device__ static inline float min(float a, float b) { return fminf(a, b); }
-    __device__ static inline float max(float a, float b) { return fmaxf(a, b); }
+    __device__ static inline float min(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -3.402823466e38f); }
+    __device__ static inline float max(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 3.402823466e38f); }
+    // three-operand forms on computed values: one v_min3 / v_max3
+    __device__ static inline float min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
+    __device__ static inline float max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
     __device__ static inline float abs(float a) { return fabsf(a); }
     __device__ static inline float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     __device__ static inline float minmod3(float a, float b, float c) {
@@ -241,19 +251,37 @@ __host__ __device__ inline V4<R> sanitise_v2(const V4<R> &q, const Params<R> &P,
     return V4<R>{vx, vy, vz, p};
 }
 
-// eos_cons_from_prim with |v|^2 of q given
+// |v|^2 capped at z_max^2 (eos.wgsl:130, 166).  The FastMath sanitise_v2
+// already returns a value <= z_max^2 (z_max^2 itself when it clamps), so the
+// cap is the identity there and is left out.
+template <class M, class R> __host__ __device__ inline R capped_v2(R v2raw, const Params<R> &P) {
+    if constexpr (M::fast) return v2raw;
+    else return M::min(v2raw, P.z_max * P.z_max);
+}
+
+// eos_cons_from_prim with |v|^2 of q given (from sanitise_v2)
 template <class M, class R>
-__host__ __device__ inline V4<R> eos_cons_from_prim_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P) {
-    const R v2 = M::min(v2raw, P.z_max * P.z_max);
-    const R w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
+__host__ __device__ inline V4<R> eos_cons_from_prim_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P,
+                                                       R &w_gam2) {
+    const R v2 = capped_v2<M>(v2raw, P);
+    w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
     return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 
-// eos_wave_speeds with |v|^2 of q given
+// eos_flux given w gamma^2 of the state as well: E + p is w gamma^2 itself,
+// which FastMath uses in place of the reference's (E + p) sum
+template <int A, class M, class R>
+__host__ __device__ inline V4<R> eos_flux_w(const V4<R> &q, const V4<R> &u, R w_gam2) {
+    V4<R> f = eos_flux<A>(q, u);
+    if constexpr (M::fast) f.x = w_gam2 * (A == 0 ? q.x : (A == 1 ? q.y : q.z));
+    return f;
+}
+
+// eos_wave_speeds with |v|^2 of q given (from sanitise_v2)
 template <int A, class M, class R>
 __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P, R &lm,
                                                    R &lp) {
-    const R v2 = M::min(v2raw, P.z_max * P.z_max);
+    const R v2 = capped_v2<M>(v2raw, P);
     const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
     const R cs2 = eos_sound_speed2(q.w, phi);
     const R cs = M::sqrt(cs2);
@@ -271,15 +299,16 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
 template <int A, class M, class R>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
-    const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P);
-    const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P);
-    const V4<R> fl = eos_flux<A>(ql, ul);
-    const V4<R> fr = eos_flux<A>(qr, ur);
+    R wl, wr;
+    const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P, wl);
+    const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P, wr);
+    const V4<R> fl = eos_flux_w<A, M>(ql, ul, wl);
+    const V4<R> fr = eos_flux_w<A, M>(qr, ur, wr);
     R llm, llp, lrm, lrp;
     eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
     eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
-    const R s_min = M::min(M::min(llm, lrm), R(0));
-    const R s_max = M::max(M::max(llp, lrp), R(0));
+    const R s_min = M::min3(llm, lrm, R(0));
+    const R s_max = M::max3(llp, lrp, R(0));
     const R den = M::max(s_max - s_min, Consts<R>::den_floor);
     const R smm = s_min * s_max;
     return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
@@ -299,8 +328,8 @@ __host__ __device__ inline V4<R> hll_flux(const V4<R> &ql, R phil, const V4<R> &
     R llm, llp, lrm, lrp;
     eos_wave_speeds<A, M>(ql, phil, P, llm, llp);
     eos_wave_speeds<A, M>(qr, phir, P, lrm, lrp);
-    const R s_min = M::min(M::min(llm, lrm), R(0));
-    const R s_max = M::max(M::max(llp, lrp), R(0));
+    const R s_min = M::min3(llm, lrm, R(0));
+    const R s_max = M::max3(llp, lrp, R(0));
     const R den = M::max(s_max - s_min, Consts<R>::den_floor);
     const R smm = s_min * s_max;
     return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
