@@ -16,6 +16,7 @@
 
 
 
+
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
@@ -124,11 +125,11 @@ __device__ inline void twopass_axis(const V2<R> *__restrict__ fin, const V4<R> *
     const V4<R> qm1 = sanitise<M>(prim[im1], P);
     const V4<R> qp1 = sanitise<M>(prim[ip1], P);
     const V4<R> qp2 = sanitise<M>(prim[ip2], P);
-    const V4<R> slope_m1 = limit_slope4<M>(sub4(qm1, qm2), sub4(q0, qm1));
-    const V4<R> slope_0 = limit_slope4<M>(sub4(q0, qm1), sub4(qp1, q0));
-    const V4<R> slope_p1 = limit_slope4<M>(sub4(qp1, q0), sub4(qp2, qp1));
-    const V4<R> f_minus = face_flux_sl<A, M>(qm1, slope_m1, q0, slope_0, phi_m, phi0, P);
-    const V4<R> f_plus = face_flux_sl<A, M>(q0, slope_0, qp1, slope_p1, phi0, phi_p, P);
+    const V4<R> h_m1 = half_slope4<M>(sub4(qm1, qm2), sub4(q0, qm1));
+    const V4<R> h_0 = half_slope4<M>(sub4(q0, qm1), sub4(qp1, q0));
+    const V4<R> h_p1 = half_slope4<M>(sub4(qp1, q0), sub4(qp2, qp1));
+    const V4<R> f_minus = face_flux_h<A, M>(qm1, h_m1, q0, h_0, phi_m, phi0, P);
+    const V4<R> f_plus = face_flux_h<A, M>(q0, h_0, qp1, h_p1, phi0, phi_p, P);
     flux_div.x = flux_div.x + (f_plus.x - f_minus.x) * P.inv_dx;
     flux_div.y = flux_div.y + (f_plus.y - f_minus.y) * P.inv_dx;
     flux_div.z = flux_div.z + (f_plus.z - f_minus.z) * P.inv_dx;
@@ -303,17 +304,6 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // in either order), so the result is the y face's, bit for bit.
     constexpr int NTAIL = TX + TY;
     const bool tail = tid < NTAIL;
-    const bool tail_y = tid >= TY;
-    int tq0 = 0, tqs = 0, tp0 = 0, tps = 0;
-    V4<R> *tout = nullptr;
-    if (tail) {
-        const int c = tail_y ? tid - TY : tid;
-        tq0 = tail_y ? TY * (TX + 4) + (c + 2) : (c + 2) * (TX + 4) + TX;
-        tqs = tail_y ? TX + 4 : 1;
-        tp0 = tail_y ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + TX;
-        tps = tail_y ? TX + 2 : 1;
-        tout = tail_y ? &L.fy[TY][c] : &L.fx[c][TX];
-    }
 
     // sanitised primitives of a cell.  eos_prim_from_cons already floors p
     // and caps |Z| <= z_max E, so |v| = |Z| / (E + p) < z_max: sanitise is
@@ -324,9 +314,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     };
 
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
-    // (phi of planes z-1, z, z+1 and the primitives of z, z+1 stay in
-    // registers; U_in of plane z is loaded again for the update, an L2 hit,
-    // which keeps the kernel at 5 waves per SIMD)
+    // (phi of plane z-1, U_in of planes z, z+1 and the primitives of z, z+1
+    // stay in registers, so every input plane is read from memory once per
+    // column)
     V2<R> f0, f1, ftmp;
     V4<R> u0, u1, utmp;
     R ph_m;
@@ -343,19 +333,20 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     f1 = plane_load(field_in + wrapi(zb + 1, P.nz) * plane, col);
     u1 = plane_load(fluid_in + wrapi(zb + 1, P.nz) * plane, col);
     V4<R> qz1 = prim_of(f1, u1);
-    const V4<R> s_b = limit_slope4<M>(sub4(qb, qa), sub4(qz, qb));
-    V4<R> sz = limit_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
-    V4<R> fzm = face_flux_sl<2, M>(qb, s_b, qz, sz, ph_m, f0.x, P);
+    const V4<R> h_b = half_slope4<M>(sub4(qb, qa), sub4(qz, qb));
+    V4<R> hz = half_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
+    V4<R> fzm = face_flux_h<2, M>(qb, h_b, qz, hz, ph_m, f0.x, P);
 
     V4<R> *qflat = &L.q[0][0];
     R *phiflat = &L.phi[0][0];
+
     for (int z = zb; z < ze; ++z) {
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
         const int64_t zoff = (int64_t)z * plane;
-        // the update's operands U_in(z) and U^n(z) of this cell, loaded first
-        // so that they arrive while the faces are computed
-        const R pi0 = plane_load(field_in + zoff, col).y;
-        const V4<R> uin = plane_load(fluid_in + zoff, col);
+        // the update's operand U^n(z) of this cell, loaded first so that it
+        // arrives while the faces are computed (U_in(z) is in the ring)
+        const R pi0 = f0.y;
+        const V4<R> uin = u0;
         V2<R> fnn{};
         V4<R> un{};
         if (stage != 0) {
@@ -387,17 +378,29 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         L.fy[ty][tx] = face_flux<1, M>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], L.q[ty + 2][tx + 2], L.q[ty + 3][tx + 2],
                                        L.phi[ty][tx + 1], L.phi[ty + 1][tx + 1], P);
         if (tail) {
-            V4<R> c0 = qflat[tq0], c1 = qflat[tq0 + tqs], c2 = qflat[tq0 + 2 * tqs], c3 = qflat[tq0 + 3 * tqs];
-            if (tail_y) {
+            // far-edge face `tid`: x faces x = TX of rows 0..TY-1, then y faces
+            // y = TY of columns 0..TX-1 (addresses formed here, in the one
+            // wave that runs this round, to keep them out of registers)
+            const bool ya = tid >= TY;
+            const int c = ya ? tid - TY : tid;
+            const int q0 = ya ? TY * (TX + 4) + (c + 2) : (c + 2) * (TX + 4) + TX;
+            const int qs = ya ? TX + 4 : 1;
+            const int p0 = ya ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + TX;
+            const int ps = ya ? TX + 2 : 1;
+            V4<R> c0 = qflat[q0], c1 = qflat[q0 + qs], c2 = qflat[q0 + 2 * qs], c3 = qflat[q0 + 3 * qs];
+            if (ya) {
                 c0 = swap_xy(c0); c1 = swap_xy(c1); c2 = swap_xy(c2); c3 = swap_xy(c3);
             }
-            V4<R> ft = face_flux<0, M>(c0, c1, c2, c3, phiflat[tp0], phiflat[tp0 + tps], P);
-            if (tail_y) ft = swap_yz(ft);
-            *tout = ft;
+            V4<R> ft = face_flux<0, M>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P);
+            if (ya) {
+                L.fy[TY][c] = swap_yz(ft);
+            } else {
+                L.fx[c][TX] = ft;
+            }
         }
         // own +z face
-        const V4<R> sz1 = limit_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
-        const V4<R> fzp = face_flux_sl<2, M>(qz, sz, qz1, sz1, f0.x, f1.x, P);
+        const V4<R> hz1 = half_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
+        const V4<R> fzp = face_flux_h<2, M>(qz, hz, qz1, hz1, f0.x, f1.x, P);
 
         // field stencil from LDS / registers (step.wgsl:118-122)
         const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
@@ -407,8 +410,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         // (d) divergence, sources, RK combination
         if (valid) {
             const R ph_0 = f0.x, ph_1 = f1.x;
-            R lap_sum = R(0);
-            lap_sum = lap_sum + (pxm + pxp - R(2) * ph_0);
+            // the reference accumulates from zero (step.wgsl:117-145); 0 + a
+            // is a itself up to the sign of a zero, which FastMath ignores
+            R lap_sum = M::fast ? (pxm + pxp - R(2) * ph_0) : R(0) + (pxm + pxp - R(2) * ph_0);
             const R gxv = R(0.5) * (pxp - pxm) * P.inv_dx;
             lap_sum = lap_sum + (pym + pyp - R(2) * ph_0);
             const R gyv = R(0.5) * (pyp - pym) * P.inv_dx;
@@ -416,11 +420,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             const R gzv = R(0.5) * (ph_1 - ph_m) * P.inv_dx;
             const V4<R> fxm = L.fx[ty][tx], fxp = L.fx[ty][tx + 1];
             const V4<R> fym = L.fy[ty][tx], fyp = L.fy[ty + 1][tx];
-            V4<R> fd{R(0), R(0), R(0), R(0)};
-            fd.x = fd.x + (fxp.x - fxm.x) * P.inv_dx;
-            fd.y = fd.y + (fxp.y - fxm.y) * P.inv_dx;
-            fd.z = fd.z + (fxp.z - fxm.z) * P.inv_dx;
-            fd.w = fd.w + (fxp.w - fxm.w) * P.inv_dx;
+            V4<R> fd{(fxp.x - fxm.x) * P.inv_dx, (fxp.y - fxm.y) * P.inv_dx, (fxp.z - fxm.z) * P.inv_dx,
+                     (fxp.w - fxm.w) * P.inv_dx};
+            if constexpr (!M::fast) fd = V4<R>{R(0) + fd.x, R(0) + fd.y, R(0) + fd.z, R(0) + fd.w};
             fd.x = fd.x + (fyp.x - fym.x) * P.inv_dx;
             fd.y = fd.y + (fyp.y - fym.y) * P.inv_dx;
             fd.z = fd.z + (fyp.z - fym.z) * P.inv_dx;
@@ -442,9 +444,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         }
 
         // (e) rotate the column ring
-        qz = qz1; qz1 = q2; sz = sz1; fzm = fzp;
+        qz = qz1; qz1 = q2; hz = hz1; fzm = fzp;
         ph_m = f0.x;
-        f0.x = f1.x; f1.x = f2.x;
+        f0 = f1; f1 = f2; u0 = u1; u1 = u2;
     }
 }
 

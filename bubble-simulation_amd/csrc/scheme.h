@@ -70,6 +70,8 @@ template <class R> struct ExactMath {
     __host__ __device__ static inline R limit_slope(R dm, R dp) {
         return minmod3(R(0.5) * (dm + dp), R(2) * dm, R(2) * dp);
     }
+    // 0.5 * limit_slope: the half slope added to q at a face (step.wgsl:136-143)
+    __host__ __device__ static inline R half_slope(R dm, R dp) { return R(0.5) * limit_slope(dm, dp); }
 };
 
 // Hardware reciprocal / square root (1 ulp each, so a*rcp(b) stays inside
@@ -100,6 +102,12 @@ struct FastMathF32 {
     // 2 med3(dm, dp, 0) -- exact, since scaling by 2 commutes with the median
     __device__ static inline float limit_slope(float dm, float dp) {
         return __builtin_amdgcn_fmed3f(0.5f * (dm + dp), 2.0f * __builtin_amdgcn_fmed3f(dm, dp, 0.0f), 0.0f);
+    }
+    // 0.5 * limit_slope with the halving moved inside the median (scaling by
+    // a power of two commutes with it): med3(0.25 (dm + dp), med3(dm, dp, 0),
+    // 0), four instructions, the same value bit for bit
+    __device__ static inline float half_slope(float dm, float dp) {
+        return __builtin_amdgcn_fmed3f(0.25f * (dm + dp), __builtin_amdgcn_fmed3f(dm, dp, 0.0f), 0.0f);
     }
 };
 
@@ -347,6 +355,12 @@ __host__ __device__ inline V4<R> limit_slope4(const V4<R> &dm, const V4<R> &dp) 
     return V4<R>{limit_slope<M>(dm.x, dp.x), limit_slope<M>(dm.y, dp.y),
                  limit_slope<M>(dm.z, dp.z), limit_slope<M>(dm.w, dp.w)};
 }
+// 0.5 * limit_slope4, per component
+template <class M, class R>
+__host__ __device__ inline V4<R> half_slope4(const V4<R> &dm, const V4<R> &dp) {
+    return V4<R>{M::half_slope(dm.x, dp.x), M::half_slope(dm.y, dp.y), M::half_slope(dm.z, dp.z),
+                 M::half_slope(dm.w, dp.w)};
+}
 template <class R> __host__ __device__ inline V4<R> sub4(const V4<R> &a, const V4<R> &b) {
     return V4<R>{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w};
 }
@@ -354,12 +368,10 @@ template <class R> __host__ __device__ inline V4<R> sub4(const V4<R> &a, const V
 // x and y momentum slots of a flux (a y face evaluated as an x face)
 template <class R> __host__ __device__ inline V4<R> swap_xy(const V4<R> &a) { return V4<R>{a.y, a.x, a.z, a.w}; }
 template <class R> __host__ __device__ inline V4<R> swap_yz(const V4<R> &a) { return V4<R>{a.x, a.z, a.y, a.w}; }
-// q + 0.5 s  and  q - 0.5 s  (step.wgsl:136-143)
-template <class R> __host__ __device__ inline V4<R> half_plus(const V4<R> &q, const V4<R> &s) {
-    return V4<R>{q.x + R(0.5) * s.x, q.y + R(0.5) * s.y, q.z + R(0.5) * s.z, q.w + R(0.5) * s.w};
-}
-template <class R> __host__ __device__ inline V4<R> half_minus(const V4<R> &q, const V4<R> &s) {
-    return V4<R>{q.x - R(0.5) * s.x, q.y - R(0.5) * s.y, q.z - R(0.5) * s.z, q.w - R(0.5) * s.w};
+// q + 0.5 s  and  q - 0.5 s  (step.wgsl:136-143), given h = 0.5 s (the
+// product by 0.5 is exact, so q + h is the reference's q + 0.5 * s)
+template <class R> __host__ __device__ inline V4<R> add4(const V4<R> &q, const V4<R> &h) {
+    return V4<R>{q.x + h.x, q.y + h.y, q.z + h.z, q.w + h.w};
 }
 
 // The flux through the face between cells L and R along axis A, given the
@@ -369,20 +381,21 @@ template <class R> __host__ __device__ inline V4<R> half_minus(const V4<R> &q, c
 template <int A, class M, class R>
 __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, const V4<R> &qr,
                                            const V4<R> &qrp, R phil, R phir, const Params<R> &P) {
-    const V4<R> sl = limit_slope4<M>(sub4(ql, qlm), sub4(qr, ql));
-    const V4<R> sr = limit_slope4<M>(sub4(qr, ql), sub4(qrp, qr));
+    const V4<R> d = sub4(qr, ql);
+    const V4<R> hl = half_slope4<M>(sub4(ql, qlm), d);
+    const V4<R> hr = half_slope4<M>(d, sub4(qrp, qr));
     R v2l, v2r;
-    const V4<R> a = sanitise_v2<M>(half_plus(ql, sl), P, v2l);
-    const V4<R> b = sanitise_v2<M>(half_minus(qr, sr), P, v2r);
+    const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
+    const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
     return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
 }
-// Same, with the two slopes already known.
+// Same, with the two half slopes (0.5 * limit_slope) already known.
 template <int A, class M, class R>
-__host__ __device__ inline V4<R> face_flux_sl(const V4<R> &ql, const V4<R> &sl, const V4<R> &qr,
-                                              const V4<R> &sr, R phil, R phir, const Params<R> &P) {
+__host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, const V4<R> &qr,
+                                             const V4<R> &hr, R phil, R phir, const Params<R> &P) {
     R v2l, v2r;
-    const V4<R> a = sanitise_v2<M>(half_plus(ql, sl), P, v2l);
-    const V4<R> b = sanitise_v2<M>(half_minus(qr, sr), P, v2r);
+    const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
+    const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
     return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
 }
 
