@@ -17,6 +17,7 @@
 
 
 
+
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
@@ -454,14 +455,11 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 // fast f32 stage fits 5 waves (<= 96 VGPRs) without spilling, and measures
 // faster there than at the 4 waves it would otherwise settle on; the exact
 // and f64 instantiations keep the compiler's choice.
-#ifndef BS_FAST_WAVES
-#define BS_FAST_WAVES 5
-#endif
 template <class M> struct Occupancy {
     static constexpr int waves = 1;
 };
 template <> struct Occupancy<FastMathF32> {
-    static constexpr int waves = BS_FAST_WAVES;
+    static constexpr int waves = 5;
 };
 
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with
