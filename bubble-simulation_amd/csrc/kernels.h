@@ -14,10 +14,6 @@
 
 #include "scheme.h"
 
-
-
-
-
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
