@@ -391,6 +391,13 @@ def device_count() -> int:
     return int(lib().bs_device_count())
 
 
+def device_memory():
+    """(free, total) bytes of the current device (hipMemGetInfo)."""
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib().bs_device_memory(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
+
+
 def device_name() -> str:
     buf = C.create_string_buffer(256)
     _check(lib().bs_device_name(buf, 256))
