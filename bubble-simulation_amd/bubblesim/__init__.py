@@ -169,6 +169,9 @@ _SIGS = {
     "bs_sim_grid": (C.c_int, [_P, _u32p]),
     "bs_sim_model": (C.c_int, [_P, C.POINTER(CModel)]),
     "bs_sim_schedule": (C.c_int, [_P, C.POINTER(CEvent), C.c_uint32, _u32p]),
+    "bs_sim_record_visualisation": (C.c_int, [_P]),
+    "bs_sim_visualisation_buffer": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint64)]),
+    "bs_sim_download_visualisation": (C.c_int, [_P, C.POINTER(C.c_uint16), C.c_uint64]),
     "bs_sim_download": (C.c_int, [_P, _P, _P]),
     "bs_sim_upload": (C.c_int, [_P, _P, _P]),
     "bs_sim_advance_timed": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -531,6 +534,23 @@ class Simulation:
         evs = (CEvent * max(n.value, 1))()
         _check(lib().bs_sim_schedule(self._h, evs, n.value, C.byref(n)))
         return events_to_array(evs, n.value)
+
+    # visualisation texels (vis.wgsl)
+    def record_visualisation(self):
+        _check(lib().bs_sim_record_visualisation(self._h))
+
+    def visualisation_buffer(self):
+        """(device address, bytes) of the texel buffer."""
+        p, n = C.c_void_p(), C.c_uint64()
+        _check(lib().bs_sim_visualisation_buffer(self._h, C.byref(p), C.byref(n)))
+        return int(p.value or 0), int(n.value)
+
+    def download_visualisation(self) -> np.ndarray:
+        """Texels as float16, shape (nz, ny, nx, 4): phi/phi_b, |v|, T/T_ref - 1, (E+p)v^2/e_ref."""
+        nx, ny, nz = self.grid
+        out = np.zeros((nz, ny, nx, 4), dtype=np.uint16)
+        _check(lib().bs_sim_download_visualisation(self._h, out.ctypes.data_as(C.POINTER(C.c_uint16)), out.size))
+        return out.view(np.float16)
 
     # state (debug dump / load)
     def download(self):

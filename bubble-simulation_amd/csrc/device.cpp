@@ -138,6 +138,17 @@ void ReduceScratch::release() {
     hres = nullptr;
 }
 
+int launch_vis(BoxState &bx, Half4 *out, hipStream_t stream) {
+    if (bx.precision == BS_PRECISION_F64) {
+        const Box<double> b = bx.box<double>();
+        BS_HIP_TRY(launch_vis_pack<double>(b.field[b.cur], b.fluid[b.cur], b.P, out, stream));
+    } else {
+        const Box<float> b = bx.box<float>();
+        BS_HIP_TRY(launch_vis_pack<float>(b.field[b.cur], b.fluid[b.cur], b.P, out, stream));
+    }
+    return BS_OK;
+}
+
 int launch_diagnostics(BoxState &bx, ReduceScratch &rs, hipStream_t stream) {
     if (bx.precision == BS_PRECISION_F64) {
         const Box<double> b = bx.box<double>();

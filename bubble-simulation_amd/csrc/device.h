@@ -84,6 +84,9 @@ struct ReduceScratch {
 int launch_diagnostics(BoxState &bx, ReduceScratch &rs, hipStream_t stream);
 void diagnostics_from_result(const ReduceResult &r, uint64_t n_cells, bs_diagnostics *out);
 
+// visualisation pack (vis.wgsl) of slot `cur` into `out` (8 B per cell)
+int launch_vis(BoxState &bx, Half4 *out, hipStream_t stream);
+
 // FNV-1a over the downloaded state of slot cur (blocking)
 int state_checksum(BoxState &bx, hipStream_t stream, uint64_t *out);
 int download_state(BoxState &bx, hipStream_t stream, void *field, void *fluid);

@@ -91,6 +91,38 @@ __global__ void __launch_bounds__(256) k_stamp(V2<R> *field, Params<R> P, StampB
 }
 
 // ---------------------------------------------------------------------------
+//  Visualisation pack (vis.wgsl:32-55, after the primitives pass that
+//  Simulation::record_visualisation runs first, sim.rs:920-942): one
+//  rgba16f texel per cell, x fastest,
+//    r = phi / phi_b, g = |v|, b = T / T_ref - 1, a = (E + p) |v|^2 / e_ref.
+//  Channels are formed in the lattice precision, narrowed to f32 (the
+//  reference's f32(...)) and stored as IEEE half, round to nearest even.
+// ---------------------------------------------------------------------------
+struct alignas(8) Half4 {
+    uint16_t r, g, b, a;
+};
+__device__ inline uint16_t half_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+template <class R>
+__global__ void __launch_bounds__(256) k_vis_pack(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
+                                                  Params<R> P, Half4 *__restrict__ out) {
+    typedef ExactMath<R> M;
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R phi = field[i].x;
+    const V4<R> u = fluid[i];
+    const V4<R> q = eos_prim_from_cons<M>(u, phi, P);  // primitives.wgsl:21
+    const R e_lab = M::max(u.x, P.e_floor);
+    const R v2 = dot3(q.x, q.y, q.z, q.x, q.y, q.z);
+    const R speed = M::sqrt(v2);
+    const R temp = eos_temperature<M>(q.w, phi, P);
+    const R kinetic = (e_lab + q.w) * v2;
+    out[i] = Half4{half_bits((float)(phi / P.phi_b)), half_bits((float)speed),
+                   half_bits((float)(temp / P.t_ref - R(1))), half_bits((float)(kinetic / P.e_ref))};
+}
+
+// ---------------------------------------------------------------------------
 //  Two-pass path: primitives (primitives.wgsl:16-23), then the stage
 //  (step.wgsl:92-174) reading the primitive cache.  Mirrors the reference's
 //  dispatch structure; faces are evaluated from both sides as there.

@@ -24,6 +24,13 @@ template <class R> hipError_t launch_stamp(V2<R> *field, const Params<R> &P, con
     return hipGetLastError();
 }
 
+template <class R>
+hipError_t launch_vis_pack(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, Half4 *out, hipStream_t st) {
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    hipLaunchKernelGGL(k_vis_pack<R>, dim3(grid_for(n, 256)), dim3(256), 0, st, field, fluid, P, out);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(64) k_reduce2(const ReducePartial *__restrict__ partials, ReduceResult *__restrict__ out) {
     // one wave folds the kReduceBlocks partials: lane l takes l, l+64, ... in
     // order, then a fixed xor-butterfly merges the lanes
@@ -210,6 +217,10 @@ template <class R> hipError_t launch_advance_multi(Box<R> *dboxes, BoxEvents *de
 template hipError_t launch_init<float>(V2<float> *, V4<float> *, int64_t, float, hipStream_t);
 template hipError_t launch_init<double>(V2<double> *, V4<double> *, int64_t, double, hipStream_t);
 template hipError_t launch_stamp<float>(V2<float> *, const Params<float> &, const StampBatch<float> &, hipStream_t);
+template hipError_t launch_vis_pack<float>(const V2<float> *, const V4<float> *, const Params<float> &, Half4 *,
+                                           hipStream_t);
+template hipError_t launch_vis_pack<double>(const V2<double> *, const V4<double> *, const Params<double> &, Half4 *,
+                                            hipStream_t);
 template hipError_t launch_stamp<double>(V2<double> *, const Params<double> &, const StampBatch<double> &, hipStream_t);
 template hipError_t launch_reduce<float>(const V2<float> *, const V4<float> *, const Params<float> &, ReducePartial *,
                                          ReduceResult *, hipStream_t);

@@ -25,6 +25,8 @@ int tile_index(TileShape t);  // -1 if not instantiated
 template <class R> hipError_t launch_init(V2<R> *field, V4<R> *fluid, int64_t n, R e_ref, hipStream_t st);
 template <class R> hipError_t launch_stamp(V2<R> *field, const Params<R> &P, const StampBatch<R> &b, hipStream_t st);
 template <class R>
+hipError_t launch_vis_pack(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, Half4 *out, hipStream_t st);
+template <class R>
 hipError_t launch_reduce(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, ReducePartial *partials,
                          ReduceResult *out, hipStream_t st);
 

@@ -18,6 +18,7 @@ struct bs_sim {
     hipStream_t stream = nullptr;
     BoxState bx;
     void *prim = nullptr;  // two-pass primitive cache
+    Half4 *vis = nullptr;  // visualisation texels, allocated on first use
     ReduceScratch rs;
     hipEvent_t diag_event = nullptr;
     bool diag_pending = false;
@@ -53,6 +54,8 @@ void sim_free(bs_sim *s) {
     s->bx.release(s->stream);
     if (s->prim) (void)hipFree(s->prim);
     s->prim = nullptr;
+    if (s->vis) (void)hipFree(s->vis);
+    s->vis = nullptr;
 }
 
 // Simulation::reset (sim.rs:790-821)
@@ -316,6 +319,7 @@ uint64_t bs_sim_device_memory_bytes(const bs_sim *s) {
     if (!s) return 0;
     uint64_t b = s->bx.bytes;
     if (s->prim) b += s->bx.n_cells * 4 * real_size(s->spec.precision);
+    if (s->vis) b += s->bx.n_cells * sizeof(Half4);
     b += sizeof(ReducePartial) * kReduceBlocks + sizeof(ReduceResult);
     return b;
 }
@@ -336,6 +340,39 @@ int bs_sim_schedule(const bs_sim *s, bs_event *out, uint32_t cap, uint32_t *n_ou
         const size_t n = std::min<size_t>(cap, s->schedule.size());
         if (n) std::memcpy(out, s->schedule.data(), n * sizeof(bs_event));
     }
+    return BS_OK;
+}
+
+// Simulation::record_visualisation (sim.rs:920-942): pack slot `cur` into
+// the texel buffer (allocated on the first call, 8 B per cell), on the
+// simulation's stream.
+int bs_sim_record_visualisation(bs_sim *s) {
+    if (!s) return set_error(BS_ERR_INVALID, "null simulation");
+    if (!s->vis) {
+        hipError_t e = hipMalloc((void **)&s->vis, s->bx.n_cells * sizeof(Half4));
+        if (e != hipSuccess) {
+            s->vis = nullptr;
+            (void)hipGetLastError();
+            return set_error(BS_ERR_OOM, "out of device memory allocating the visualisation texels");
+        }
+    }
+    return launch_vis(s->bx, s->vis, s->stream);
+}
+
+int bs_sim_visualisation_buffer(bs_sim *s, void **device_ptr, uint64_t *bytes) {
+    if (!s || !device_ptr) return set_error(BS_ERR_INVALID, "null argument");
+    if (!s->vis) return set_error(BS_ERR_NOT_READY, "no visualisation recorded yet");
+    *device_ptr = s->vis;
+    if (bytes) *bytes = s->bx.n_cells * sizeof(Half4);
+    return BS_OK;
+}
+
+int bs_sim_download_visualisation(bs_sim *s, uint16_t *out, uint64_t n_halfs) {
+    if (!s || !out) return set_error(BS_ERR_INVALID, "null argument");
+    if (!s->vis) return set_error(BS_ERR_NOT_READY, "no visualisation recorded yet");
+    if (n_halfs < 4 * s->bx.n_cells) return set_error(BS_ERR_INVALID, "output holds fewer than 4 halves per cell");
+    BS_HIP_TRY(hipMemcpyAsync(out, s->vis, s->bx.n_cells * sizeof(Half4), hipMemcpyDeviceToHost, s->stream));
+    BS_HIP_TRY(hipStreamSynchronize(s->stream));
     return BS_OK;
 }
 

@@ -336,6 +336,18 @@ int bs_sim_grid(const bs_sim *sim, uint32_t grid_out[3]);
 int bs_sim_model(const bs_sim *sim, bs_model *out);
 /* Copy of the schedule (remaining + already stamped), n_out = total. */
 int bs_sim_schedule(const bs_sim *sim, bs_event *out, uint32_t cap, uint32_t *n_out);
+/* Simulation::record_visualisation (sim.rs:920-942; vis.wgsl:32-55): pack
+ * slot `cur` into one rgba16f texel per cell, x fastest, channels
+ * (phi/phi_b, |v|, T/T_ref - 1, (E+p)|v|^2/e_ref) as IEEE half bits, round
+ * to nearest even.  The texel buffer (8 B per cell) is allocated on the
+ * first call and counted by bs_sim_device_memory_bytes; it replaces the
+ * reference's vis_tex storage texture.  Enqueued on the simulation's stream. */
+int bs_sim_record_visualisation(bs_sim *sim);
+/* Device address and size of the texel buffer, for a renderer on the same
+ * device (BS_ERR_NOT_READY before the first record_visualisation). */
+int bs_sim_visualisation_buffer(bs_sim *sim, void **device_ptr, uint64_t *bytes);
+/* Blocking copy of the texels: 4 halves per cell (n_halfs >= 4 * cells). */
+int bs_sim_download_visualisation(bs_sim *sim, uint16_t *out, uint64_t n_halfs);
 /* Debug dump / load of U^n (slot `cur`), layout in the header comment. */
 int bs_sim_download(bs_sim *sim, void *field, void *fluid);
 int bs_sim_upload(bs_sim *sim, const void *field, const void *fluid);

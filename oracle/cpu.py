@@ -44,6 +44,7 @@ def lib():
         L.oracle_set_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_diagnostics.argtypes = [C.c_void_p, f64p]
         L.oracle_diagnostics_naive_f32.argtypes = [C.c_void_p, f32p, C.POINTER(C.c_uint32)]
+        L.oracle_vis_pack.argtypes = [C.c_void_p, f32p]
         L.oracle_prim_from_cons.argtypes = [C.c_int, f32p, f64p, C.c_double, f64p]
         L.oracle_cons_from_prim.argtypes = [f32p, f64p, f64p]
         L.oracle_wave_speeds.argtypes = [f32p, f64p, C.c_int, f64p]
@@ -89,6 +90,7 @@ class OracleSim:
         self._e, ep = _f32(np.asarray(events, dtype=np.float32).reshape(-1, 5))
         u = self._p.view(np.uint32)
         self.grid = (int(u[0]), int(u[1]), int(u[2]))
+        self.n = self.grid[0] * self.grid[1] * self.grid[2]
         self._h = lib().oracle_create(precision, pp, ep, len(self._e))
 
     def close(self):
@@ -131,6 +133,12 @@ class OracleSim:
         """(sum e_total, sum broken, sum kinetic, sum E, max|v|), long double sums."""
         out = np.zeros(5, dtype=np.float64)
         lib().oracle_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def vis_channels(self) -> np.ndarray:
+        """vis.wgsl's four channels per cell, narrowed to f32: shape (n, 4)."""
+        out = np.zeros((self.n, 4), dtype=np.float32)
+        lib().oracle_vis_pack(self._h, out.ctypes.data_as(C.POINTER(C.c_float)))
         return out
 
     def diagnostics_naive_f32(self):

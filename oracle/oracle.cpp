@@ -14,6 +14,7 @@
 //   shaders/init.wgsl       initial condition
 //   shaders/nucleate.wgsl   min_image, tanh profile, max-merge
 //   shaders/reduce.wgsl     cell_contribution; naive f32 two-pass summation
+//   shaders/vis.wgsl        visualisation channels (after the primitive pass)
 //   src/gpu/sim.rs          RK3 (:135), stage_slots (:1046-1051), record_steps /
 //                           record_nucleation / record_step (:829-916)
 //
@@ -432,6 +433,33 @@ template <class R> Lattice<R> *make_lattice(const float *params, const float *ev
     return L;
 }
 
+// eos_temperature (eos.wgsl:65-69): T = (3 max(p, p_floor) / a)^(1/4)
+template <class R> inline R eos_temperature(const Prm<R> &P, R p, R /*phi*/) {
+    const R x = R(3) * wmax(p, P.p_floor) / P.a_rad;
+    return std::sqrt(std::sqrt(x));
+}
+
+// record_visualisation (sim.rs:920-942): primitives.wgsl on slot cur, then
+// vis.wgsl:38-55 per cell; the four channels narrowed to f32 (f32(...)),
+// before the texture's own conversion to half
+template <class R> void vis_pack(const Lattice<R> &L, float *out) {
+    const Prm<R> &P = L.P;
+    for (int64_t i = 0; i < L.n; ++i) {
+        const R phi = L.field[L.cur][2 * i];
+        const P4<R> u = L.fl(L.cur, i);
+        const P4<R> q = eos_prim_from_cons(P, u, phi);
+        const R e_lab = wmax(u.x, P.e_floor);
+        const R v2 = wdot(q, q);
+        const R speed = std::sqrt(v2);
+        const R temp = eos_temperature(P, q.w, phi);
+        const R kinetic = (e_lab + q.w) * v2;
+        out[4 * i] = (float)(phi / P.phi_b);
+        out[4 * i + 1] = (float)speed;
+        out[4 * i + 2] = (float)(temp / P.t_ref - R(1));
+        out[4 * i + 3] = (float)(kinetic / P.e_ref);
+    }
+}
+
 template <class R> void diag_exact(Lattice<R> &L, double *out) {
     // per-cell contributions in R; sums in long double
     L.primitives(L.cur);
@@ -582,6 +610,13 @@ void oracle_diagnostics(void *hp, double *out) {
     Handle *h = (Handle *)hp;
     if (h->ld) diag_exact(*h->ld, out);
     else diag_exact(*h->lf, out);
+}
+
+// visualisation channels of slot cur, 4 floats per cell
+void oracle_vis_pack(void *hp, float *out) {
+    Handle *h = (Handle *)hp;
+    if (h->ld) vis_pack(*h->ld, out);
+    else vis_pack(*h->lf, out);
 }
 
 // reduce.wgsl's own f32 result (f32 lattices only)
