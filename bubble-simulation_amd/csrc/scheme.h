@@ -292,37 +292,77 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
     const R v2 = capped_v2<M>(v2raw, P);
     const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
     const R cs2 = eos_sound_speed2(q.w, phi);
-    const R cs = M::sqrt(cs2);
     const R denom = R(1) - v2 * cs2;
-    const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
-    const R a = M::div(vd * (R(1) - cs2), denom);
-    const R b = M::div(cs * M::sqrt(rad), denom);
-    lm = a - b;
-    lp = a + b;
+    if constexpr (M::fast) {
+        // cs sqrt(rad) as sqrt(cs^2 rad); rad > 0 without the clamp, since
+        // v_d^2 <= |v|^2 <= z_max^2 < 1 after sanitise
+        const R rad_c = (R(1) - v2) * ((R(1) - vd * vd - cs2 * (v2 - vd * vd)) * cs2);
+        const R rd = M::div(R(1), denom);
+        const R a = vd * (R(1) - cs2) * rd;
+        const R b = M::sqrt(rad_c) * rd;
+        lm = a - b;
+        lp = a + b;
+    } else {
+        const R cs = M::sqrt(cs2);
+        const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
+        const R a = M::div(vd * (R(1) - cs2), denom);
+        const R b = M::div(cs * M::sqrt(rad), denom);
+        lm = a - b;
+        lp = a + b;
+    }
 }
 
 // hll_flux (step.wgsl:77-90) along axis A, with |v|^2 of both states given
 // (the reference recomputes dot(v, v) in eos_cons_from_prim and
 // eos_wave_speeds; the value is the same).
+//
+// FastMath evaluates the same flux factored by side.  With w = w gamma^2,
+// U = (w - p, w v), F = (w v_d, w v v_d + p e_d):
+//   F_HLL = (s+ F_L - s- F_R + s- s+ (U_R - U_L)) / (s+ - s-)
+//         = (s+ A - s- B) / (s+ - s-),   A = F_L - s- U_L,  B = F_R - s+ U_R,
+// and with t_L = w_L (v_d,L - s-), t_R = w_R (v_d,R - s+):
+//   A = (t_L + s- p_L, v_L t_L + p_L e_d),  B = (t_R + s+ p_R, v_R t_R + p_R e_d),
+// one product per component and side instead of forming U and F.
 template <int A, class M, class R>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
-    R wl, wr;
-    const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P, wl);
-    const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P, wr);
-    const V4<R> fl = eos_flux_w<A, M>(ql, ul, wl);
-    const V4<R> fr = eos_flux_w<A, M>(qr, ur, wr);
-    R llm, llp, lrm, lrp;
-    eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
-    eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
-    const R s_min = M::min3(llm, lrm, R(0));
-    const R s_max = M::max3(llp, lrp, R(0));
-    const R den = M::max(s_max - s_min, Consts<R>::den_floor);
-    const R smm = s_min * s_max;
-    return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
-                 M::div(s_max * fl.y - s_min * fr.y + smm * (ur.y - ul.y), den),
-                 M::div(s_max * fl.z - s_min * fr.z + smm * (ur.z - ul.z), den),
-                 M::div(s_max * fl.w - s_min * fr.w + smm * (ur.w - ul.w), den)};
+    if constexpr (M::fast) {
+        const R wl = M::div(eos_enthalpy_from_pressure(ql.w, phil), R(1) - capped_v2<M>(v2l, P));
+        const R wr = M::div(eos_enthalpy_from_pressure(qr.w, phir), R(1) - capped_v2<M>(v2r, P));
+        R llm, llp, lrm, lrp;
+        eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
+        eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
+        const R s_min = M::min3(llm, lrm, R(0));
+        const R s_max = M::max3(llp, lrp, R(0));
+        const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
+        const R cmax = s_max * rden, cmin = s_min * rden;
+        const R vdl = A == 0 ? ql.x : (A == 1 ? ql.y : ql.z);
+        const R vdr = A == 0 ? qr.x : (A == 1 ? qr.y : qr.z);
+        const R tl = wl * (vdl - s_min), tr = wr * (vdr - s_max);
+        const V4<R> a{tl + s_min * ql.w, ql.x * tl + (A == 0 ? ql.w : R(0)), ql.y * tl + (A == 1 ? ql.w : R(0)),
+                      ql.z * tl + (A == 2 ? ql.w : R(0))};
+        const V4<R> b{tr + s_max * qr.w, qr.x * tr + (A == 0 ? qr.w : R(0)), qr.y * tr + (A == 1 ? qr.w : R(0)),
+                      qr.z * tr + (A == 2 ? qr.w : R(0))};
+        return V4<R>{cmax * a.x - cmin * b.x, cmax * a.y - cmin * b.y, cmax * a.z - cmin * b.z,
+                     cmax * a.w - cmin * b.w};
+    } else {
+        R wl, wr;
+        const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P, wl);
+        const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P, wr);
+        const V4<R> fl = eos_flux_w<A, M>(ql, ul, wl);
+        const V4<R> fr = eos_flux_w<A, M>(qr, ur, wr);
+        R llm, llp, lrm, lrp;
+        eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
+        eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
+        const R s_min = M::min3(llm, lrm, R(0));
+        const R s_max = M::max3(llp, lrp, R(0));
+        const R den = M::max(s_max - s_min, Consts<R>::den_floor);
+        const R smm = s_min * s_max;
+        return V4<R>{M::div(s_max * fl.x - s_min * fr.x + smm * (ur.x - ul.x), den),
+                     M::div(s_max * fl.y - s_min * fr.y + smm * (ur.y - ul.y), den),
+                     M::div(s_max * fl.z - s_min * fr.z + smm * (ur.z - ul.z), den),
+                     M::div(s_max * fl.w - s_min * fr.w + smm * (ur.w - ul.w), den)};
+    }
 }
 
 // hll_flux (step.wgsl:77-90) along axis A.
