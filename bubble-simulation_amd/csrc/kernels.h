@@ -275,7 +275,7 @@ template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, uns
     __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(base, idx));
 }
 
-template <int TX, int TY, class M, class R>
+template <int TX, int TY, class M, class R, bool S0>
 __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
                                            const V2<R> *__restrict__ field_n, const V4<R> *__restrict__ fluid_n,
                                            const V2<R> *__restrict__ field_in, const V4<R> *__restrict__ fluid_in,
@@ -376,9 +376,12 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         // arrives while the faces are computed (U_in(z) is in the ring)
         const R pi0 = f0.y;
         const V4<R> uin = u0;
-        V2<R> fnn{};
-        V4<R> un{};
-        if (stage != 0) {
+        V2<R> fnn;
+        V4<R> un;
+        if constexpr (S0) {
+            fnn = V2<R>{f0.x, pi0};  // U_in is U^n at stage 0 (same slot)
+            un = uin;
+        } else {
             fnn = stream_load(field_n + zoff, col);
             un = stream_load(fluid_n + zoff, col);
         }
@@ -439,14 +442,23 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         // (d) divergence, sources, RK combination
         if (valid) {
             const R ph_0 = f0.x, ph_1 = f1.x;
-            // the reference accumulates from zero (step.wgsl:117-145); 0 + a
-            // is a itself up to the sign of a zero, which FastMath ignores
-            R lap_sum = M::fast ? (pxm + pxp - R(2) * ph_0) : R(0) + (pxm + pxp - R(2) * ph_0);
-            const R gxv = R(0.5) * (pxp - pxm) * P.inv_dx;
-            lap_sum = lap_sum + (pym + pyp - R(2) * ph_0);
-            const R gyv = R(0.5) * (pyp - pym) * P.inv_dx;
-            lap_sum = lap_sum + (ph_m + ph_1 - R(2) * ph_0);
-            const R gzv = R(0.5) * (ph_1 - ph_m) * P.inv_dx;
+            R lap_sum, gxv, gyv, gzv;
+            if constexpr (M::fast) {
+                // the same sums regrouped, and 0.5 / dx as one factor
+                lap_sum = ((pxm + pxp) + (pym + pyp)) + (ph_m + ph_1) - R(6) * ph_0;
+                const R hdx = R(0.5) * P.inv_dx;
+                gxv = (pxp - pxm) * hdx;
+                gyv = (pyp - pym) * hdx;
+                gzv = (ph_1 - ph_m) * hdx;
+            } else {
+                // the reference accumulates from zero (step.wgsl:117-145)
+                lap_sum = R(0) + (pxm + pxp - R(2) * ph_0);
+                gxv = R(0.5) * (pxp - pxm) * P.inv_dx;
+                lap_sum = lap_sum + (pym + pyp - R(2) * ph_0);
+                gyv = R(0.5) * (pyp - pym) * P.inv_dx;
+                lap_sum = lap_sum + (ph_m + ph_1 - R(2) * ph_0);
+                gzv = R(0.5) * (ph_1 - ph_m) * P.inv_dx;
+            }
             const V4<R> fxm = L.fx[ty][tx], fxp = L.fx[ty][tx + 1];
             const V4<R> fym = L.fy[ty][tx], fyp = L.fy[ty + 1][tx];
             V4<R> fd{(fxp.x - fxm.x) * P.inv_dx, (fxp.y - fxm.y) * P.inv_dx, (fxp.z - fxm.z) * P.inv_dx,
@@ -460,14 +472,10 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
             fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
             fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
-            R phn = fnn.x, pin = fnn.y;
-            if (stage == 0) {
-                phn = ph_0; pin = pi0; un = uin;   // U_in is U^n at stage 0 (same slot)
-            }
             R phi_o, pi_o;
             V4<R> fo;
-            stage_update<M>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, phn, pin, un, a0, a1, dt, P,
-                            phi_o, pi_o, fo);
+            stage_update<M, R, S0>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, fnn.x, fnn.y, un, a0, a1, dt, P,
+                                   phi_o, pi_o, fo);
             stream_store(field_out + zoff, col, V2<R>{phi_o, pi_o});
             stream_store(fluid_out + zoff, col, fo);
         }
@@ -494,7 +502,7 @@ template <> struct Occupancy<FastMathF32> {
 // tile_x fastest; consecutive blocks land on different XCDs (round-robin
 // dispatch), so the logical tile of block b is remapped such that each XCD
 // works on a contiguous run of tiles and halo re-reads hit its own L2.
-template <int TX, int TY, class M, class R>
+template <int TX, int TY, class M, class R, bool S0>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc, int nblocks) {
     __shared__ FusedLds<TX, TY, R> L;
     int b = blockIdx.x;
@@ -507,7 +515,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int in, out;
     stage_slot_ids(box.cur, stage, in, out);
     const int n = box.cur;
-    fused_tile<TX, TY, M, R>(L, box.P, box.field[n], box.fluid[n], box.field[in], box.fluid[in], box.field[out],
+    fused_tile<TX, TY, M, R, S0>(L, box.P, box.field[n], box.fluid[n], box.field[in], box.fluid[in], box.field[out],
                              box.fluid[out], tx_i * TX, ty_i * TY, zb, ze, stage);
 }
 
@@ -529,7 +537,7 @@ struct BoxEvents {
     int active;
     int pad;
 };
-template <int TX, int TY, class M, class R>
+template <int TX, int TY, class M, class R, bool S0>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused_multi(const Box<R> *__restrict__ boxes,
                                                               const BoxEvents *__restrict__ events,
                                                               const Task *__restrict__ tasks, int stage, int ntasks) {
@@ -543,7 +551,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int in, out;
     stage_slot_ids(bx.cur, stage, in, out);
     const int n = bx.cur;
-    fused_tile<TX, TY, M, R>(L, bx.P, bx.field[n], bx.fluid[n], bx.field[in], bx.fluid[in], bx.field[out],
+    fused_tile<TX, TY, M, R, S0>(L, bx.P, bx.field[n], bx.fluid[n], bx.field[in], bx.fluid[in], bx.field[out],
                              bx.fluid[out], t.x0, t.y0, t.zb, t.ze, stage);
 }
 

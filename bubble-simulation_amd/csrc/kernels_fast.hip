@@ -12,14 +12,14 @@ static hipError_t fused_one(const Box<float> &box, int stage, int zc, hipStream_
     const int nty = (box.P.ny + TY - 1) / TY;
     const int nzc = (box.P.nz + zc - 1) / zc;
     const int nblocks = ntx * nty * nzc;
-    hipLaunchKernelGGL((k_stage_fused<TX, TY, FastMathF32, float>), dim3(nblocks), dim3(TX * TY), 0, st, box, stage,
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused<TX, TY, FastMathF32, float, true> : k_stage_fused<TX, TY, FastMathF32, float, false>), dim3(nblocks), dim3(TX * TY), 0, st, box, stage,
                        ntx, nty, zc, nblocks);
     return hipGetLastError();
 }
 
 template <int TX, int TY>
 static hipError_t fused_multi_one(const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage, hipStream_t st) {
-    hipLaunchKernelGGL((k_stage_fused_multi<TX, TY, FastMathF32, float>), dim3(ntasks), dim3(TX * TY), 0, st, dboxes,
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, FastMathF32, float, true> : k_stage_fused_multi<TX, TY, FastMathF32, float, false>), dim3(ntasks), dim3(TX * TY), 0, st, dboxes,
                        devents, dtasks, stage, ntasks);
     return hipGetLastError();
 }

@@ -161,7 +161,9 @@ template <class M, class R> __host__ __device__ inline R eos_temperature(R p, R 
 // 3p^2 + 2Ep - (E^2 - Z^2) = 0.
 template <class M, class R>
 __host__ __device__ inline R eos_pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
-    const R disc = M::max(R(4) * e_lab * e_lab - R(3) * z2, R(0));
+    // disc >= E^2 (4 - 3 z_max^2) > 0 once |Z| <= z_max E; FastMath drops
+    // the clamp
+    const R disc = M::fast ? R(4) * e_lab * e_lab - R(3) * z2 : M::max(R(4) * e_lab * e_lab - R(3) * z2, R(0));
     const R p = M::div(e_lab * e_lab - z2, e_lab + M::sqrt(disc));
     return M::max(p, P.p_floor);
 }
@@ -187,7 +189,7 @@ __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params
     }
     const R p = eos_pressure_from_conserved<M>(e_lab, dot3(zx, zy, zz, zx, zy, zz), phi, P);
     const R s = e_lab + p;
-    const R sm = M::max(s, Consts<R>::tiny);
+    const R sm = M::fast ? s : M::max(s, Consts<R>::tiny);  // s >= e_floor + p_floor > 0
     return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
 }
 
@@ -445,13 +447,16 @@ __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, c
 // Given phi0, pi0, the sanitised primitive q0, the Laplacian sum, the
 // gradient, the flux divergence (already scaled by inv_dx), and U_in / U^n of
 // the cell, produce U_out.
-template <class M, class R>
+// S0: the first stage (a0 = 0, a1 = 1), where FastMath takes the combination
+// as U_in + dt RHS itself; the exact policy keeps 0 * U^n + 1 * (...), whose
+// zero signs follow the reference.
+template <class M, class R, bool S0 = false>
 __host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R lap_sum, R gx, R gy, R gz,
                                              const V4<R> &flux_div, const V4<R> &fin,
                                              R phin, R pin, const V4<R> &fn,
                                              R a0, R a1, R dt, const Params<R> &P,
                                              R &phi_out, R &pi_out, V4<R> &f_out) {
-    const R laplacian = lap_sum * P.inv_dx * P.inv_dx;
+    const R laplacian = M::fast ? lap_sum * (P.inv_dx * P.inv_dx) : lap_sum * P.inv_dx * P.inv_dx;
     const R gam = M::inv_sqrt(M::max(R(1) - dot3(q0.x, q0.y, q0.z, q0.x, q0.y, q0.z), Consts<R>::den_floor));
     const R dphi_proper = gam * (pi0 + dot3(q0.x, q0.y, q0.z, gx, gy, gz));
     const R drag = P.eta * dphi_proper;
@@ -465,9 +470,15 @@ __host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R l
     const R up_phi = phi0 + dt * d_phi;
     const R up_pi = pi0 + dt * d_pi;
     const V4<R> up{fin.x + dt * d.x, fin.y + dt * d.y, fin.z + dt * d.z, fin.w + dt * d.w};
-    phi_out = a0 * phin + a1 * up_phi;
-    pi_out = a0 * pin + a1 * up_pi;
-    f_out = V4<R>{a0 * fn.x + a1 * up.x, a0 * fn.y + a1 * up.y, a0 * fn.z + a1 * up.z, a0 * fn.w + a1 * up.w};
+    if constexpr (M::fast && S0) {
+        phi_out = up_phi;
+        pi_out = up_pi;
+        f_out = up;
+    } else {
+        phi_out = a0 * phin + a1 * up_phi;
+        pi_out = a0 * pin + a1 * up_pi;
+        f_out = V4<R>{a0 * fn.x + a1 * up.x, a0 * fn.y + a1 * up.y, a0 * fn.z + a1 * up.z, a0 * fn.w + a1 * up.w};
+    }
 }
 
 // ---------------------------------------------------------------------------
