@@ -14,6 +14,7 @@
 
 #include "scheme.h"
 
+
 namespace bs {
 
 // One lattice box as the kernels see it.  `field[s]`, `fluid[s]` are the
@@ -198,11 +199,14 @@ __global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__
 //  [zb, ze) and marches up in z.  Per plane:
 //    - the sanitised primitives of the plane, with a 2-cell halo in x and y,
 //      sit in LDS (interior ones come from registers, where they were
-//      computed two planes earlier; halo ones are recomputed here);
-//    - every x- and y-face of the tile is evaluated once (face items are
-//      spread over all threads) and stored in LDS;
+//      computed two planes earlier; halo ones from a thread-private LDS
+//      ring, recovered two planes earlier together with the column load of
+//      that plane, while the neighbouring tiles load the same plane);
+//    - every x- and y-face of the tile is evaluated once (one face per
+//      thread and axis, then one round for the far-edge faces) and stored
+//      in LDS;
 //    - each thread evaluates its own +z face from a register ring of the
-//      column's primitives q(z..z+2), slope(z) and carries the -z face;
+//      column's primitives q(z..z+2), half slope(z) and carries the -z face;
 //    - the divergence, friction and RK combination are applied per cell.
 //  The result is bitwise that of k_prim + k_stage_twopass under ExactMath:
 //  every face sees the same operands in the same order.
@@ -212,6 +216,8 @@ template <int TX, int TY, class R> struct FusedLds {
     R phi[TY + 2][TX + 2];    // phi of plane z, 1-cell halo
     V4<R> fx[TY][TX + 1];     // flux through the face left of tile column c
     V4<R> fy[TY + 1][TX];     // flux through the face below tile row r
+    V4<R> hq[3][4 * (TX + TY)];  // halo primitives of planes z .. z+2 (each slot private to its thread)
+    R hph[3][4 * (TX + TY)];     // their phi
 };
 
 template <class R> __device__ inline int64_t cidx(const Params<R> &P, int x, int y, int z) {
@@ -368,6 +374,23 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 
     V4<R> *qflat = &L.q[0][0];
     R *phiflat = &L.phi[0][0];
+    // halo cells are recovered two planes ahead, together with the column
+    // load of the same plane, and wait in a thread-private LDS ring
+    auto halo_ahead = [&](int zz, int slot) {
+        const int64_t zo = (int64_t)zz * plane;
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (hlds[k] >= 0) {
+                const int h = tid + k * NT;
+                const V2<R> fh = plane_load(field_in + zo, hoff[k]);
+                const V4<R> uh = plane_load(fluid_in + zo, hoff[k]);
+                L.hq[slot][h] = prim_of(fh, uh);
+                L.hph[slot][h] = fh.x;
+            }
+        }
+    };
+    halo_ahead(zb, zb % 3);
+    if (zb + 1 < ze) halo_ahead(wrapi(zb + 1, P.nz), (zb + 1) % 3);
 
     for (int z = zb; z < ze; ++z) {
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
@@ -389,17 +412,17 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         const V2<R> f2 = plane_load(field_in + zp2 * plane, col);
         const V4<R> u2 = plane_load(fluid_in + zp2 * plane, col);
         const V4<R> q2 = prim_of(f2, u2);
+        if (z + 2 < ze) halo_ahead(zp2, (z + 2) % 3);
 
-        // (b) plane z into LDS: interior from registers, halo recomputed
+        // (b) plane z into LDS: interior from registers, halo from the ring
         L.q[ty + 2][tx + 2] = qz;
         L.phi[ty + 1][tx + 1] = f0.x;
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                const V2<R> fh = plane_load(field_in + zoff, hoff[k]);
-                const V4<R> uh = plane_load(fluid_in + zoff, hoff[k]);
-                qflat[hlds[k]] = prim_of(fh, uh);
-                if (hphi[k] >= 0) phiflat[hphi[k]] = fh.x;
+                const int h = tid + k * NT;
+                qflat[hlds[k]] = L.hq[z % 3][h];
+                if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[z % 3][h];
             }
         }
         __syncthreads();
