@@ -29,6 +29,7 @@ CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "bubble-sim")
 BS_OK, BS_ERR_INVALID, BS_ERR_OOM, BS_ERR_DEVICE, BS_ERR_NONFINITE, BS_ERR_NOT_READY, BS_ERR_NOT_FOUND = range(7)
 PRECISION_F32, PRECISION_F64 = 0, 1
 MATH_FAST, MATH_EXACT = 0, 1
+EOS_BAG, EOS_GAMMA_LAW = 0, 1
 KERNEL_FUSED, KERNEL_TWO_PASS = 0, 1
 SEED_CRITICAL, SEED_CELLS = 0, 1
 NUC_SIMULTANEOUS, NUC_EXPONENTIAL, NUC_UNIFORM_STEPS = 0, 1, 2
@@ -93,7 +94,8 @@ class CSpec(C.Structure):
     _fields_ = [("model", CModel), ("grid", C.c_uint32 * 3), ("nucleation", C.c_int32), ("bubbles", C.c_uint32),
                 ("nucleation_duration", C.c_float), ("seed", C.c_uint64), ("precision", C.c_int32),
                 ("math", C.c_int32), ("kernel", C.c_int32), ("bubble_at_centre", C.c_int32),
-                ("uniform_max_step", C.c_uint32), ("pad0", C.c_uint32), ("time_seed", C.c_uint64)]
+                ("uniform_max_step", C.c_uint32), ("eos", C.c_int32), ("time_seed", C.c_uint64),
+                ("eos_gamma", C.c_float), ("dof_ratio", C.c_float)]
 
 
 class CServerConfig(C.Structure):
@@ -335,7 +337,7 @@ def default_spec() -> CSpec:
 def make_spec(grid=192, model: Optional[Model] = None, nucleation: int = NUC_EXPONENTIAL, bubbles: int = 12,
               duration: float = 60.0, seed: int = 1, precision: int = PRECISION_F32, math: int = MATH_FAST,
               kernel: int = KERNEL_FUSED, bubble_at_centre: bool = False, uniform_max_step: int = 200,
-              time_seed: int = 0x5EED) -> CSpec:
+              time_seed: int = 0x5EED, eos: int = 0, eos_gamma: float = 4.0 / 3.0, dof_ratio: float = 1.0) -> CSpec:
     s = default_spec()
     s.model = (model or Model()).to_c()
     g = _grid3(grid)
@@ -350,6 +352,9 @@ def make_spec(grid=192, model: Optional[Model] = None, nucleation: int = NUC_EXP
     s.bubble_at_centre = 1 if bubble_at_centre else 0
     s.uniform_max_step = uniform_max_step
     s.time_seed = time_seed
+    s.eos = eos
+    s.eos_gamma = eos_gamma
+    s.dof_ratio = dof_ratio
     return s
 
 

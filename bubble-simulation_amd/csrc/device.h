@@ -25,11 +25,13 @@ namespace bs {
 // f32 parameter block -> lattice-precision Params (exact widening for f64)
 template <class R> inline Params<R> widen_params(const bs_sim_params &p) {
     Params<R> q;
-    q.nx = (int)p.nx; q.ny = (int)p.ny; q.nz = (int)p.nz; q.pad = 0;
+    q.nx = (int)p.nx; q.ny = (int)p.ny; q.nz = (int)p.nz; q.eos = kEosBag;
     q.dx = (R)p.dx; q.inv_dx = (R)p.inv_dx; q.dt = (R)p.dt; q.eta = (R)p.eta;
     q.lambda = (R)p.lambda; q.eps = (R)p.eps; q.phi_b = (R)p.phi_b; q.m2 = (R)p.m2;
     q.delta = (R)p.delta; q.a_rad = (R)p.a_rad; q.wall_width = (R)p.wall_width; q.p_floor = (R)p.p_floor;
     q.e_floor = (R)p.e_floor; q.z_max = (R)p.z_max; q.e_ref = (R)p.e_ref; q.t_ref = (R)p.t_ref;
+    q.eos_gamma = (R)(4.0f / 3.0f);
+    q.dof_ratio = R(1);
     return q;
 }
 
@@ -46,6 +48,10 @@ struct BoxState {
     int cur = 0;
     uint64_t bytes = 0;
     bool async_alloc = false;
+    int eos = BS_EOS_BAG;               // equation of state (bs_spec.eos)
+    float eos_gamma = 4.0f / 3.0f;
+    float dof_ratio = 1.0f;
+    void set_eos(const bs_spec &s) { eos = s.eos; eos_gamma = s.eos_gamma; dof_ratio = s.dof_ratio; }
 
     // allocate the slots (stream-ordered when `stream` is given)
     int alloc(int prec, const uint32_t g[3], hipStream_t stream, bool async);
@@ -54,6 +60,9 @@ struct BoxState {
     template <class R> Box<R> box() const {
         Box<R> b;
         b.P = widen_params<R>(params);
+        b.P.eos = eos == BS_EOS_GAMMA_LAW ? kEosGammaLaw : kEosBag;
+        b.P.eos_gamma = (R)eos_gamma;
+        b.P.dof_ratio = (R)dof_ratio;
         for (int s = 0; s < 3; ++s) {
             b.field[s] = (V2<R> *)field[s];
             b.fluid[s] = (V4<R> *)fluid[s];

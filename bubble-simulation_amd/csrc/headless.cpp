@@ -62,6 +62,9 @@ void usage(FILE *f) {
                  "      --bubble-at-centre             One bubble at the box centre at t = 0\n"
                  "      --uniform-max-step <N>         uniform-steps nucleation: steps in [0, N] [default: 200]\n"
                  "      --time-seed <SEED>             uniform-steps nucleation time generator seed\n"
+                 "      --eos <bag|gamma-law>          Equation of state [default: bag]\n"
+                 "      --eos-gamma <G>                gamma-law adiabatic index, 1 < G < 2 [default: 4/3]\n"
+                 "      --dof-ratio <R>                a_broken / a_symmetric of the temperature [default: 1]\n"
                  "      --dump <PATH>                  Write the final state (field then fluid, raw) to PATH\n"
                  "  -h, --help                         Print help\n"
                  "  -V, --version                      Print version\n");
@@ -134,7 +137,8 @@ int main(int argc, char **argv) {
             if (bs_parse_grid(v.c_str(), spec.grid) != BS_OK)
                 return fail_usage("invalid value '" + v + "' for '--grid <GRID>': " + bs_last_error());
         } else if (a == "--lambda" || a == "--eps-ratio" || a == "--alpha" || a == "--eta" || a == "--cfl" ||
-                   a == "--seed-factor" || a == "--seed-radius" || a == "--nucleation-duration") {
+                   a == "--seed-factor" || a == "--seed-radius" || a == "--nucleation-duration" ||
+                   a == "--eos-gamma" || a == "--dof-ratio") {
             float f;
             if (!parse_f32(v, f)) return fail_usage("invalid value '" + v + "' for '" + a + "': invalid float literal");
             if (a == "--lambda") spec.model.lambda = f;
@@ -144,6 +148,8 @@ int main(int argc, char **argv) {
             else if (a == "--cfl") spec.model.cfl = f;
             else if (a == "--seed-factor") seed_factor = f;
             else if (a == "--seed-radius") { have_seed_radius = true; seed_radius = f; }
+            else if (a == "--eos-gamma") spec.eos_gamma = f;
+            else if (a == "--dof-ratio") spec.dof_ratio = f;
             else spec.nucleation_duration = f;
         } else if (a == "--bubbles" || a == "--seed" || a == "--steps-per-frame" || a == "--headless" ||
                    a == "--report-every" || a == "--uniform-max-step" || a == "--time-seed") {
@@ -179,6 +185,10 @@ int main(int argc, char **argv) {
             if (v == "fused") spec.kernel = BS_KERNEL_FUSED;
             else if (v == "two-pass") spec.kernel = BS_KERNEL_TWO_PASS;
             else return fail_usage("invalid value '" + v + "' for '--kernel'");
+        } else if (a == "--eos") {
+            if (v == "bag") spec.eos = BS_EOS_BAG;
+            else if (v == "gamma-law") spec.eos = BS_EOS_GAMMA_LAW;
+            else return fail_usage("invalid value '" + v + "' for '--eos'");
         } else if (a == "--dump") {
             dump_path = v;
         } else {

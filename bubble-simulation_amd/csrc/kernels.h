@@ -104,7 +104,7 @@ struct alignas(8) Half4 {
 };
 __device__ inline uint16_t half_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
-template <class R>
+template <class R, class E>
 __global__ void __launch_bounds__(256) k_vis_pack(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
                                                   Params<R> P, Half4 *__restrict__ out) {
     typedef ExactMath<R> M;
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(256) k_vis_pack(const V2<R> *__restrict__ fiel
     if (i >= n) return;
     const R phi = field[i].x;
     const V4<R> u = fluid[i];
-    const V4<R> q = eos_prim_from_cons<M>(u, phi, P);  // primitives.wgsl:21
+    const V4<R> q = eos_prim_from_cons<M, R, E>(u, phi, P);  // primitives.wgsl:21
     const R e_lab = M::max(u.x, P.e_floor);
     const R v2 = dot3(q.x, q.y, q.z, q.x, q.y, q.z);
     const R speed = M::sqrt(v2);
@@ -128,16 +128,16 @@ __global__ void __launch_bounds__(256) k_vis_pack(const V2<R> *__restrict__ fiel
 //  (step.wgsl:92-174) reading the primitive cache.  Mirrors the reference's
 //  dispatch structure; faces are evaluated from both sides as there.
 // ---------------------------------------------------------------------------
-template <class M, class R>
+template <class M, class R, class E>
 __global__ void __launch_bounds__(256) k_prim(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
                                               V4<R> *__restrict__ prim, Params<R> P) {
     const int64_t n = (int64_t)P.nx * P.ny * P.nz;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    prim[i] = eos_prim_from_cons<M>(fluid[i], field[i].x, P);
+    prim[i] = eos_prim_from_cons<M, R, E>(fluid[i], field[i].x, P);
 }
 
-template <int A, class M, class R>
+template <int A, class M, class R, class E>
 __device__ inline void twopass_axis(const V2<R> *__restrict__ fin, const V4<R> *__restrict__ prim, const Params<R> &P,
                                     int x, int y, int z, R phi0, const V4<R> &q0, R &lap_sum, R &g, V4<R> &flux_div) {
     auto idx = [&](int dd) -> int64_t {
@@ -158,15 +158,15 @@ __device__ inline void twopass_axis(const V2<R> *__restrict__ fin, const V4<R> *
     const V4<R> h_m1 = half_slope4<M>(sub4(qm1, qm2), sub4(q0, qm1));
     const V4<R> h_0 = half_slope4<M>(sub4(q0, qm1), sub4(qp1, q0));
     const V4<R> h_p1 = half_slope4<M>(sub4(qp1, q0), sub4(qp2, qp1));
-    const V4<R> f_minus = face_flux_h<A, M>(qm1, h_m1, q0, h_0, phi_m, phi0, P);
-    const V4<R> f_plus = face_flux_h<A, M>(q0, h_0, qp1, h_p1, phi0, phi_p, P);
+    const V4<R> f_minus = face_flux_h<A, M, R, E>(qm1, h_m1, q0, h_0, phi_m, phi0, P);
+    const V4<R> f_plus = face_flux_h<A, M, R, E>(q0, h_0, qp1, h_p1, phi0, phi_p, P);
     flux_div.x = flux_div.x + (f_plus.x - f_minus.x) * P.inv_dx;
     flux_div.y = flux_div.y + (f_plus.y - f_minus.y) * P.inv_dx;
     flux_div.z = flux_div.z + (f_plus.z - f_minus.z) * P.inv_dx;
     flux_div.w = flux_div.w + (f_plus.w - f_minus.w) * P.inv_dx;
 }
 
-template <class M, class R>
+template <class M, class R, class E>
 __global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__ field_n, const V4<R> *__restrict__ fluid_n,
                                                        const V2<R> *__restrict__ field_in, const V4<R> *__restrict__ fluid_in,
                                                        const V4<R> *__restrict__ prim, V2<R> *__restrict__ field_out,
@@ -182,9 +182,9 @@ __global__ void __launch_bounds__(256) k_stage_twopass(const V2<R> *__restrict__
     const V4<R> q0 = sanitise<M>(prim[i0], P);
     R lap_sum = R(0), gx, gy, gz;
     V4<R> flux_div{R(0), R(0), R(0), R(0)};
-    twopass_axis<0, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gx, flux_div);
-    twopass_axis<1, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gy, flux_div);
-    twopass_axis<2, M>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gz, flux_div);
+    twopass_axis<0, M, R, E>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gx, flux_div);
+    twopass_axis<1, M, R, E>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gy, flux_div);
+    twopass_axis<2, M, R, E>(field_in, prim, P, x, y, z, phi0, q0, lap_sum, gz, flux_div);
     const V2<R> fnn = field_n[i0];
     R phi_o, pi_o;
     V4<R> fo;
@@ -281,7 +281,7 @@ template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, uns
     __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(base, idx));
 }
 
-template <int TX, int TY, class M, class R, bool S0>
+template <int TX, int TY, class M, class R, bool S0, class E>
 __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
                                            const V2<R> *__restrict__ field_n, const V4<R> *__restrict__ fluid_n,
                                            const V2<R> *__restrict__ field_in, const V4<R> *__restrict__ fluid_in,
@@ -344,8 +344,8 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // and caps |Z| <= z_max E, so |v| = |Z| / (E + p) < z_max: sanitise is
     // the identity up to rounding, and FastMath leaves it out.
     auto prim_of = [&](const V2<R> &f, const V4<R> &u) -> V4<R> {
-        if constexpr (M::fast) return eos_prim_from_cons<M>(u, f.x, P);
-        else return sanitise<M>(eos_prim_from_cons<M>(u, f.x, P), P);
+        if constexpr (M::fast) return eos_prim_from_cons<M, R, E>(u, f.x, P);
+        else return sanitise<M>(eos_prim_from_cons<M, R, E>(u, f.x, P), P);
     };
 
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
@@ -370,7 +370,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     V4<R> qz1 = prim_of(f1, u1);
     const V4<R> h_b = half_slope4<M>(sub4(qb, qa), sub4(qz, qb));
     V4<R> hz = half_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
-    V4<R> fzm = face_flux_h<2, M>(qb, h_b, qz, hz, ph_m, f0.x, P);
+    V4<R> fzm = face_flux_h<2, M, R, E>(qb, h_b, qz, hz, ph_m, f0.x, P);
 
     V4<R> *qflat = &L.q[0][0];
     R *phiflat = &L.phi[0][0];
@@ -428,9 +428,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         __syncthreads();
 
         // (c) x and y faces, slopes formed inside each face from four cells
-        L.fx[ty][tx] = face_flux<0, M>(L.q[ty + 2][tx], L.q[ty + 2][tx + 1], L.q[ty + 2][tx + 2], L.q[ty + 2][tx + 3],
+        L.fx[ty][tx] = face_flux<0, M, R, E>(L.q[ty + 2][tx], L.q[ty + 2][tx + 1], L.q[ty + 2][tx + 2], L.q[ty + 2][tx + 3],
                                        L.phi[ty + 1][tx], L.phi[ty + 1][tx + 1], P);
-        L.fy[ty][tx] = face_flux<1, M>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], L.q[ty + 2][tx + 2], L.q[ty + 3][tx + 2],
+        L.fy[ty][tx] = face_flux<1, M, R, E>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], L.q[ty + 2][tx + 2], L.q[ty + 3][tx + 2],
                                        L.phi[ty][tx + 1], L.phi[ty + 1][tx + 1], P);
         if (tail) {
             // far-edge face `tid`: x faces x = TX of rows 0..TY-1, then y faces
@@ -446,7 +446,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             if (ya) {
                 c0 = swap_xy(c0); c1 = swap_xy(c1); c2 = swap_xy(c2); c3 = swap_xy(c3);
             }
-            V4<R> ft = face_flux<0, M>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P);
+            V4<R> ft = face_flux<0, M, R, E>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P);
             if (ya) {
                 L.fy[TY][c] = swap_yz(ft);
             } else {
@@ -455,7 +455,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         }
         // own +z face
         const V4<R> hz1 = half_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
-        const V4<R> fzp = face_flux_h<2, M>(qz, hz, qz1, hz1, f0.x, f1.x, P);
+        const V4<R> fzp = face_flux_h<2, M, R, E>(qz, hz, qz1, hz1, f0.x, f1.x, P);
 
         // field stencil from LDS / registers (step.wgsl:118-122)
         const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
@@ -525,7 +525,7 @@ template <> struct Occupancy<FastMathF32> {
 // tile_x fastest; consecutive blocks land on different XCDs (round-robin
 // dispatch), so the logical tile of block b is remapped such that each XCD
 // works on a contiguous run of tiles and halo re-reads hit its own L2.
-template <int TX, int TY, class M, class R, bool S0>
+template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc, int nblocks) {
     __shared__ FusedLds<TX, TY, R> L;
     int b = blockIdx.x;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int in, out;
     stage_slot_ids(box.cur, stage, in, out);
     const int n = box.cur;
-    fused_tile<TX, TY, M, R, S0>(L, box.P, box.field[n], box.fluid[n], box.field[in], box.fluid[in], box.field[out],
+    fused_tile<TX, TY, M, R, S0, E>(L, box.P, box.field[n], box.fluid[n], box.field[in], box.fluid[in], box.field[out],
                              box.fluid[out], tx_i * TX, ty_i * TY, zb, ze, stage);
 }
 
@@ -560,7 +560,7 @@ struct BoxEvents {
     int active;
     int pad;
 };
-template <int TX, int TY, class M, class R, bool S0>
+template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused_multi(const Box<R> *__restrict__ boxes,
                                                               const BoxEvents *__restrict__ events,
                                                               const Task *__restrict__ tasks, int stage, int ntasks) {
@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int in, out;
     stage_slot_ids(bx.cur, stage, in, out);
     const int n = bx.cur;
-    fused_tile<TX, TY, M, R, S0>(L, bx.P, bx.field[n], bx.fluid[n], bx.field[in], bx.fluid[in], bx.field[out],
+    fused_tile<TX, TY, M, R, S0, E>(L, bx.P, bx.field[n], bx.fluid[n], bx.field[in], bx.fluid[in], bx.field[out],
                              bx.fluid[out], t.x0, t.y0, t.zb, t.ze, stage);
 }
 
@@ -609,7 +609,7 @@ struct ReducePartial {
     double nonfinite;
 };
 
-template <class R>
+template <class R, class E>
 __global__ void __launch_bounds__(kReduceThreads) k_reduce1(const V2<R> *__restrict__ field, const V4<R> *__restrict__ fluid,
                                                             Params<R> P, ReducePartial *__restrict__ partials) {
     __shared__ ReducePartial sh[kReduceThreads / 64];
@@ -627,7 +627,7 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce1(const V2<R> *__restr
         const R pym = field[cidx(P, x, wrapi(y - 1, P.ny), z)].x, pyp = field[cidx(P, x, wrapi(y + 1, P.ny), z)].x;
         const R pzm = field[cidx(P, x, y, wrapi(z - 1, P.nz))].x, pzp = field[cidx(P, x, y, wrapi(z + 1, P.nz))].x;
         const CellDiag<ExactMath<R>, R> c =
-            cell_contribution<ExactMath<R>, R>(f.x, f.y, pxm, pxp, pym, pyp, pzm, pzp, fluid[i], P);
+            cell_contribution<ExactMath<R>, R, E>(f.x, f.y, pxm, pxp, pym, pyp, pzm, pzp, fluid[i], P);
         neumaier_add(acc[0], (double)c.e_total);
         neumaier_add(acc[1], (double)c.broken);
         neumaier_add(acc[2], (double)c.kinetic);

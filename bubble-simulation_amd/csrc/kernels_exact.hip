@@ -27,7 +27,10 @@ template <class R> hipError_t launch_stamp(V2<R> *field, const Params<R> &P, con
 template <class R>
 hipError_t launch_vis_pack(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, Half4 *out, hipStream_t st) {
     const int64_t n = (int64_t)P.nx * P.ny * P.nz;
-    hipLaunchKernelGGL(k_vis_pack<R>, dim3(grid_for(n, 256)), dim3(256), 0, st, field, fluid, P, out);
+    if (P.eos == kEosGammaLaw)
+        hipLaunchKernelGGL((k_vis_pack<R, GammaLawEos>), dim3(grid_for(n, 256)), dim3(256), 0, st, field, fluid, P, out);
+    else
+        hipLaunchKernelGGL((k_vis_pack<R, BagEos>), dim3(grid_for(n, 256)), dim3(256), 0, st, field, fluid, P, out);
     return hipGetLastError();
 }
 
@@ -62,40 +65,52 @@ __global__ void __launch_bounds__(64) k_reduce2(const ReducePartial *__restrict_
 template <class R>
 hipError_t launch_reduce(const V2<R> *field, const V4<R> *fluid, const Params<R> &P, ReducePartial *partials,
                          ReduceResult *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce1<R>, dim3(kReduceBlocks), dim3(kReduceThreads), 0, st, field, fluid, P, partials);
+    if (P.eos == kEosGammaLaw)
+        hipLaunchKernelGGL((k_reduce1<R, GammaLawEos>), dim3(kReduceBlocks), dim3(kReduceThreads), 0, st, field, fluid, P,
+                           partials);
+    else
+        hipLaunchKernelGGL((k_reduce1<R, BagEos>), dim3(kReduceBlocks), dim3(kReduceThreads), 0, st, field, fluid, P,
+                           partials);
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(64), 0, st, partials, out);
     return hipGetLastError();
 }
 
-template <int TX, int TY, class M, class R>
+template <int TX, int TY, class M, class R, class E>
 static hipError_t fused_one(const Box<R> &box, int stage, int zc, hipStream_t st) {
     const int ntx = (box.P.nx + TX - 1) / TX;
     const int nty = (box.P.ny + TY - 1) / TY;
     const int nzc = (box.P.nz + zc - 1) / zc;
     const int nblocks = ntx * nty * nzc;
-    hipLaunchKernelGGL((stage == 0 ? k_stage_fused<TX, TY, M, R, true> : k_stage_fused<TX, TY, M, R, false>), dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc,
-                       nblocks);
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused<TX, TY, M, R, true, E> : k_stage_fused<TX, TY, M, R, false, E>),
+                       dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc, nblocks);
     return hipGetLastError();
 }
 
 template <int TX, int TY, class M, class R>
 static hipError_t fused_multi_one(const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage, hipStream_t st) {
-    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, M, R, true> : k_stage_fused_multi<TX, TY, M, R, false>), dim3(ntasks), dim3(TX * TY), 0, st, dboxes, devents, dtasks, stage,
-                       ntasks);
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, M, R, true, BagEos>
+                                   : k_stage_fused_multi<TX, TY, M, R, false, BagEos>),
+                       dim3(ntasks), dim3(TX * TY), 0, st, dboxes, devents, dtasks, stage, ntasks);
     return hipGetLastError();
 }
 
+// The bag model is instantiated for every tile shape; the gamma-law EOS for
+// the two default shapes (32x8, 16x16), other requests run on 32x8.
 template <class M, class R>
 static hipError_t fused_dispatch(const Box<R> &box, int stage, TileShape t, int zc, hipStream_t st) {
+    if (box.P.eos == kEosGammaLaw) {
+        if (tile_index(t) == 3) return fused_one<16, 16, M, R, GammaLawEos>(box, stage, zc, st);
+        return fused_one<32, 8, M, R, GammaLawEos>(box, stage, zc, st);
+    }
     switch (tile_index(t)) {
-    case 0: return fused_one<32, 16, M, R>(box, stage, zc, st);
-    case 1: return fused_one<64, 8, M, R>(box, stage, zc, st);
-    case 2: return fused_one<32, 8, M, R>(box, stage, zc, st);
-    case 3: return fused_one<16, 16, M, R>(box, stage, zc, st);
-    case 4: return fused_one<64, 4, M, R>(box, stage, zc, st);
-    case 5: return fused_one<16, 8, M, R>(box, stage, zc, st);
-    case 6: return fused_one<32, 4, M, R>(box, stage, zc, st);
-    case 7: return fused_one<16, 15, M, R>(box, stage, zc, st);
+    case 0: return fused_one<32, 16, M, R, BagEos>(box, stage, zc, st);
+    case 1: return fused_one<64, 8, M, R, BagEos>(box, stage, zc, st);
+    case 2: return fused_one<32, 8, M, R, BagEos>(box, stage, zc, st);
+    case 3: return fused_one<16, 16, M, R, BagEos>(box, stage, zc, st);
+    case 4: return fused_one<64, 4, M, R, BagEos>(box, stage, zc, st);
+    case 5: return fused_one<16, 8, M, R, BagEos>(box, stage, zc, st);
+    case 6: return fused_one<32, 4, M, R, BagEos>(box, stage, zc, st);
+    case 7: return fused_one<16, 15, M, R, BagEos>(box, stage, zc, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -136,19 +151,24 @@ hipError_t launch_stage_fused_multi<double>(int, const Box<double> *dboxes, cons
     return fused_multi_dispatch<ExactMath<double>, double>(dboxes, devents, dtasks, ntasks, stage, t, st);
 }
 
-template <class M, class R>
-static hipError_t twopass(const Box<R> &box, int stage, V4<R> *prim, hipStream_t st) {
+template <class M, class R, class E>
+static hipError_t twopass_e(const Box<R> &box, int stage, V4<R> *prim, hipStream_t st) {
     const int64_t n = (int64_t)box.P.nx * box.P.ny * box.P.nz;
     int in, out;
     stage_slot_ids(box.cur, stage, in, out);
     R a0, a1;
     RK3c<R>::get(stage, a0, a1);
-    hipLaunchKernelGGL((k_prim<M, R>), dim3(grid_for(n, 256)), dim3(256), 0, st, box.field[in], box.fluid[in], prim,
+    hipLaunchKernelGGL((k_prim<M, R, E>), dim3(grid_for(n, 256)), dim3(256), 0, st, box.field[in], box.fluid[in], prim,
                        box.P);
-    hipLaunchKernelGGL((k_stage_twopass<M, R>), dim3(grid_for(n, 256)), dim3(256), 0, st, box.field[box.cur],
+    hipLaunchKernelGGL((k_stage_twopass<M, R, E>), dim3(grid_for(n, 256)), dim3(256), 0, st, box.field[box.cur],
                        box.fluid[box.cur], box.field[in], box.fluid[in], prim, box.field[out], box.fluid[out], box.P, a0,
                        a1);
     return hipGetLastError();
+}
+template <class M, class R>
+static hipError_t twopass(const Box<R> &box, int stage, V4<R> *prim, hipStream_t st) {
+    if (box.P.eos == kEosGammaLaw) return twopass_e<M, R, GammaLawEos>(box, stage, prim, st);
+    return twopass_e<M, R, BagEos>(box, stage, prim, st);
 }
 template <>
 hipError_t launch_stage_twopass<float>(int math, const Box<float> &box, int stage, V4<float> *prim, hipStream_t st) {

@@ -13,6 +13,7 @@ int Entry::create(const bs_spec &s, hipStream_t stream, bool async) {
     std::string err;
     int rc = spec_validate(s, err);
     if (rc) return set_error(rc, err);
+    if (s.eos != BS_EOS_BAG) return set_error(BS_ERR_INVALID, "the batched engine runs the bag equation of state only");
     rc = spec_schedule(s, schedule, err);
     if (rc) return set_error(rc, err);
     rc = bx.alloc(s.precision, s.grid, stream, async);

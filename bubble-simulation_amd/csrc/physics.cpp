@@ -350,6 +350,22 @@ int spec_validate(const bs_spec &s, std::string &err) {
         err = "unknown nucleation mode";
         return BS_ERR_INVALID;
     }
+    if (s.eos != BS_EOS_BAG && s.eos != BS_EOS_GAMMA_LAW) {
+        err = "eos must be BS_EOS_BAG or BS_EOS_GAMMA_LAW";
+        return BS_ERR_INVALID;
+    }
+    if (s.eos == BS_EOS_GAMMA_LAW && !(s.eos_gamma > 1.0f && s.eos_gamma < 2.0f)) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf,
+                      "eos_gamma = %g is outside (1, 2): the sound speed squared, eos_gamma - 1, must lie in (0, 1)",
+                      (double)s.eos_gamma);
+        err = buf;
+        return BS_ERR_INVALID;
+    }
+    if (!(s.dof_ratio > 0.0f) || !std::isfinite(s.dof_ratio)) {
+        err = "dof_ratio must be positive";
+        return BS_ERR_INVALID;
+    }
     return BS_OK;
 }
 
@@ -367,6 +383,9 @@ void spec_default(bs_spec *s) {
     s->bubble_at_centre = 0;
     s->uniform_max_step = 200;
     s->time_seed = 0x5eedULL;
+    s->eos = BS_EOS_BAG;
+    s->eos_gamma = 4.0f / 3.0f;
+    s->dof_ratio = 1.0f;
 }
 
 int parse_grid(const char *s, uint32_t g[3], std::string &err) {

@@ -25,12 +25,16 @@ template <class R> struct alignas(2 * sizeof(R)) V2 { R x, y; };
 // f32 bs_sim_params; for R = double every value is the exact widening of the
 // f32 one, so the FP64 scheme runs the same physics in wider arithmetic.
 template <class R> struct Params {
-    int nx, ny, nz, pad;
+    int nx, ny, nz, eos;  // eos: kEosBag / kEosGammaLaw
     R dx, inv_dx, dt, eta;
     R lambda, eps, phi_b, m2;
     R delta, a_rad, wall_width, p_floor;
     R e_floor, z_max, e_ref, t_ref;
+    R eos_gamma;  // adiabatic index of the gamma-law equation of state
+    R dof_ratio;  // a_broken / a_symmetric for the temperature (docs/EOS.md:65-84)
 };
+constexpr int kEosBag = 0;
+constexpr int kEosGammaLaw = 1;
 
 template <class R> struct Consts;
 template <> struct Consts<float> {
@@ -143,34 +147,97 @@ template <class R> __host__ __device__ inline R dpotential(R phi, const Params<R
 }
 
 // ---------------------------------------------------------------------------
-//  Bag equation of state (eos.wgsl) -- swap point #2
+//  Equation of state (eos.wgsl) -- swap point #2
+//
+//  An equation of state is a policy class with the scalar functions of
+//  docs/EOS.md:12-23 (each takes phi, the hook for a phase-dependent EOS);
+//  the vector functions below (prim_from_cons, cons_from_prim, flux, wave
+//  speeds) are written once on top of it, from the two structural facts the
+//  solver relies on: E = w W^2 - p, Z = w W^2 v.
 // ---------------------------------------------------------------------------
-// eos_sound_speed2 (eos.wgsl:50-54): 1/3 for e = 3p.
-template <class R> __host__ __device__ inline R eos_sound_speed2(R /*p*/, R /*phi*/) { return R(1) / R(3); }
-// eos_enthalpy_from_pressure (eos.wgsl:56-59): w = 4p.
-template <class R> __host__ __device__ inline R eos_enthalpy_from_pressure(R p, R /*phi*/) { return R(4) * p; }
-// eos_energy_from_pressure (eos.wgsl:61-63): e = 3p.
-template <class R> __host__ __device__ inline R eos_energy_from_pressure(R p, R /*phi*/) { return R(3) * p; }
-// eos_temperature (eos.wgsl:65-69): T = (3 max(p, p_floor) / a)^(1/4).
-template <class M, class R> __host__ __device__ inline R eos_temperature(R p, R /*phi*/, const Params<R> &P) {
-    const R x = R(3) * M::max(p, P.p_floor) / P.a_rad;
-    return M::sqrt(M::sqrt(x));
-}
+// The bag model, e = 3p (eos.wgsl:50-98): closed-form inversion.
+struct BagEos {
+    static constexpr int kind = kEosBag;
+    // eos_sound_speed2 (eos.wgsl:50-54): 1/3 for e = 3p.
+    template <class R> __host__ __device__ static inline R sound_speed2(R /*p*/, R /*phi*/, const Params<R> &) {
+        return R(1) / R(3);
+    }
+    // eos_enthalpy_from_pressure (eos.wgsl:56-59): w = 4p.
+    template <class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &) {
+        return R(4) * p;
+    }
+    // eos_energy_from_pressure (eos.wgsl:61-63): e = 3p.
+    template <class R> __host__ __device__ static inline R energy(R p, R /*phi*/, const Params<R> &) {
+        return R(3) * p;
+    }
+    // eos_pressure_from_conserved (eos.wgsl:94-98): conjugate root of
+    // 3p^2 + 2Ep - (E^2 - Z^2) = 0.
+    template <class M, class R>
+    __host__ __device__ static inline R pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
+        // disc >= E^2 (4 - 3 z_max^2) > 0 once |Z| <= z_max E; FastMath drops
+        // the clamp
+        const R disc = M::fast ? R(4) * e_lab * e_lab - R(3) * z2 : M::max(R(4) * e_lab * e_lab - R(3) * z2, R(0));
+        const R p = M::div(e_lab * e_lab - z2, e_lab + M::sqrt(disc));
+        return M::max(p, P.p_floor);
+    }
+};
 
-// eos_pressure_from_conserved (eos.wgsl:94-98): conjugate root of
-// 3p^2 + 2Ep - (E^2 - Z^2) = 0.
-template <class M, class R>
-__host__ __device__ inline R eos_pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
-    // disc >= E^2 (4 - 3 z_max^2) > 0 once |Z| <= z_max E; FastMath drops
-    // the clamp
-    const R disc = M::fast ? R(4) * e_lab * e_lab - R(3) * z2 : M::max(R(4) * e_lab * e_lab - R(3) * z2, R(0));
-    const R p = M::div(e_lab * e_lab - z2, e_lab + M::sqrt(disc));
-    return M::max(p, P.p_floor);
+// A conformal gamma-law fluid, p = (Gamma - 1) e (docs/EOS.md:86-128, worked
+// example 2): c_s^2 = Gamma - 1, w = Gamma p / (Gamma - 1).  Gamma = 4/3 is
+// the bag model.  The inversion iterates, as the document describes: with
+// s = E + p the rest-frame energy is e(p) = E - Z^2 / s, and Newton's method
+// solves f(p) = p - (Gamma - 1) e(p) = 0 with f'(p) = 1 - (Gamma - 1) Z^2 / s^2.
+// f is increasing and convex on p > -E, so from p0 = (Gamma - 1) E (the root
+// at Z = 0, above every root) the iterates decrease monotonically onto the
+// root; the count is fixed (kIters), never a tolerance loop, so one bad cell
+// cannot stall a wave.
+struct GammaLawEos {
+    static constexpr int kind = kEosGammaLaw;
+    static constexpr int kIters = 8;
+    template <class R> __host__ __device__ static inline R sound_speed2(R /*p*/, R /*phi*/, const Params<R> &P) {
+        return P.eos_gamma - R(1);
+    }
+    template <class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &P) {
+        return P.eos_gamma * p / (P.eos_gamma - R(1));
+    }
+    template <class R> __host__ __device__ static inline R energy(R p, R /*phi*/, const Params<R> &P) {
+        return p / (P.eos_gamma - R(1));
+    }
+    template <class M, class R>
+    __host__ __device__ static inline R pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
+        const R gm1 = P.eos_gamma - R(1);
+        R p = gm1 * e_lab;
+        for (int it = 0; it < kIters; ++it) {
+            const R s = e_lab + p;
+            const R rs = M::div(R(1), s);
+            const R zs = z2 * rs;                        // Z^2 / s
+            const R f = p - gm1 * (e_lab - zs);
+            const R df = R(1) - gm1 * (zs * rs);         // 1 - (Gamma - 1) Z^2 / s^2
+            p = p - M::div(f, df);
+        }
+        return M::max(p, P.p_floor);
+    }
+};
+
+// eos_temperature (eos.wgsl:65-69): T = (3 max(p, p_floor) / a)^(1/4), with
+// the radiation constant interpolated across the wall when the two phases
+// have different degrees of freedom (docs/EOS.md:65-84, worked example 1):
+// a(phi) = a (1 + (dof_ratio - 1) h^2 (3 - 2h)), h = clamp(phi / phi_b, 0, 1).
+// dof_ratio = 1 is the reference's expression, operation for operation.
+template <class M, class R> __host__ __device__ inline R eos_temperature(R p, R phi, const Params<R> &P) {
+    R a = P.a_rad;
+    if (P.dof_ratio != R(1)) {
+        const R h = M::min(M::max(phi / P.phi_b, R(0)), R(1));
+        const R smooth = h * h * (R(3) - R(2) * h);
+        a = P.a_rad * (R(1) + (P.dof_ratio - R(1)) * smooth);
+    }
+    const R x = R(3) * M::max(p, P.p_floor) / a;
+    return M::sqrt(M::sqrt(x));
 }
 
 // eos_prim_from_cons with eos_regularise inlined (eos.wgsl:102-123):
 // (E, Z) -> (v, p).
-template <class M, class R>
+template <class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params<R> &P) {
     const R e_lab = M::max(u.x, P.e_floor);
     R zx = u.y, zy = u.z, zz = u.w;
@@ -187,17 +254,17 @@ __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params
         const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
         zx = zx * f; zy = zy * f; zz = zz * f;
     }
-    const R p = eos_pressure_from_conserved<M>(e_lab, dot3(zx, zy, zz, zx, zy, zz), phi, P);
+    const R p = E::template pressure_from_conserved<M>(e_lab, dot3(zx, zy, zz, zx, zy, zz), phi, P);
     const R s = e_lab + p;
     const R sm = M::fast ? s : M::max(s, Consts<R>::tiny);  // s >= e_floor + p_floor > 0
     return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
 }
 
 // eos_cons_from_prim (eos.wgsl:129-135).
-template <class M, class R>
+template <class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> eos_cons_from_prim(const V4<R> &q, R phi, const Params<R> &P) {
     const R v2 = M::min(dot3(q.x, q.y, q.z, q.x, q.y, q.z), P.z_max * P.z_max);
-    const R w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
+    const R w_gam2 = M::div(E::enthalpy(q.w, phi, P), R(1) - v2);
     return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 
@@ -214,11 +281,11 @@ __host__ __device__ inline V4<R> eos_flux(const V4<R> &q, const V4<R> &u) {
 }
 
 // eos_wave_speeds (eos.wgsl:164-176): relativistic (lambda-, lambda+).
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline void eos_wave_speeds(const V4<R> &q, R phi, const Params<R> &P, R &lm, R &lp) {
     const R v2 = M::min(dot3(q.x, q.y, q.z, q.x, q.y, q.z), P.z_max * P.z_max);
     const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
-    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs2 = E::sound_speed2(q.w, phi, P);
     const R cs = M::sqrt(cs2);
     const R denom = R(1) - v2 * cs2;
     const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
@@ -270,11 +337,11 @@ template <class M, class R> __host__ __device__ inline R capped_v2(R v2raw, cons
 }
 
 // eos_cons_from_prim with |v|^2 of q given (from sanitise_v2)
-template <class M, class R>
+template <class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> eos_cons_from_prim_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P,
                                                        R &w_gam2) {
     const R v2 = capped_v2<M>(v2raw, P);
-    w_gam2 = M::div(eos_enthalpy_from_pressure(q.w, phi), R(1) - v2);
+    w_gam2 = M::div(E::enthalpy(q.w, phi, P), R(1) - v2);
     return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 
@@ -288,12 +355,12 @@ __host__ __device__ inline V4<R> eos_flux_w(const V4<R> &q, const V4<R> &u, R w_
 }
 
 // eos_wave_speeds with |v|^2 of q given (from sanitise_v2)
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P, R &lm,
                                                    R &lp) {
     const R v2 = capped_v2<M>(v2raw, P);
     const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
-    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs2 = E::sound_speed2(q.w, phi, P);
     const R denom = R(1) - v2 * cs2;
     if constexpr (M::fast) {
         // cs sqrt(rad) as sqrt(cs^2 rad); rad > 0 without the clamp, since
@@ -325,15 +392,15 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
 // and with t_L = w_L (v_d,L - s-), t_R = w_R (v_d,R - s+):
 //   A = (t_L + s- p_L, v_L t_L + p_L e_d),  B = (t_R + s+ p_R, v_R t_R + p_R e_d),
 // one product per component and side instead of forming U and F.
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
     if constexpr (M::fast) {
-        const R wl = M::div(eos_enthalpy_from_pressure(ql.w, phil), R(1) - capped_v2<M>(v2l, P));
-        const R wr = M::div(eos_enthalpy_from_pressure(qr.w, phir), R(1) - capped_v2<M>(v2r, P));
+        const R wl = M::div(E::enthalpy(ql.w, phil, P), R(1) - capped_v2<M>(v2l, P));
+        const R wr = M::div(E::enthalpy(qr.w, phir, P), R(1) - capped_v2<M>(v2r, P));
         R llm, llp, lrm, lrp;
-        eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
-        eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
+        eos_wave_speeds_v2<A, M, R, E>(ql, v2l, phil, P, llm, llp);
+        eos_wave_speeds_v2<A, M, R, E>(qr, v2r, phir, P, lrm, lrp);
         const R s_min = M::min3(llm, lrm, R(0));
         const R s_max = M::max3(llp, lrp, R(0));
         const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
@@ -349,13 +416,13 @@ __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, con
                      cmax * a.w - cmin * b.w};
     } else {
         R wl, wr;
-        const V4<R> ul = eos_cons_from_prim_v2<M>(ql, v2l, phil, P, wl);
-        const V4<R> ur = eos_cons_from_prim_v2<M>(qr, v2r, phir, P, wr);
+        const V4<R> ul = eos_cons_from_prim_v2<M, R, E>(ql, v2l, phil, P, wl);
+        const V4<R> ur = eos_cons_from_prim_v2<M, R, E>(qr, v2r, phir, P, wr);
         const V4<R> fl = eos_flux_w<A, M>(ql, ul, wl);
         const V4<R> fr = eos_flux_w<A, M>(qr, ur, wr);
         R llm, llp, lrm, lrp;
-        eos_wave_speeds_v2<A, M>(ql, v2l, phil, P, llm, llp);
-        eos_wave_speeds_v2<A, M>(qr, v2r, phir, P, lrm, lrp);
+        eos_wave_speeds_v2<A, M, R, E>(ql, v2l, phil, P, llm, llp);
+        eos_wave_speeds_v2<A, M, R, E>(qr, v2r, phir, P, lrm, lrp);
         const R s_min = M::min3(llm, lrm, R(0));
         const R s_max = M::max3(llp, lrp, R(0));
         const R den = M::max(s_max - s_min, Consts<R>::den_floor);
@@ -368,16 +435,16 @@ __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, con
 }
 
 // hll_flux (step.wgsl:77-90) along axis A.
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> hll_flux(const V4<R> &ql, R phil, const V4<R> &qr, R phir,
                                           const Params<R> &P) {
-    const V4<R> ul = eos_cons_from_prim<M>(ql, phil, P);
-    const V4<R> ur = eos_cons_from_prim<M>(qr, phir, P);
+    const V4<R> ul = eos_cons_from_prim<M, R, E>(ql, phil, P);
+    const V4<R> ur = eos_cons_from_prim<M, R, E>(qr, phir, P);
     const V4<R> fl = eos_flux<A>(ql, ul);
     const V4<R> fr = eos_flux<A>(qr, ur);
     R llm, llp, lrm, lrp;
-    eos_wave_speeds<A, M>(ql, phil, P, llm, llp);
-    eos_wave_speeds<A, M>(qr, phir, P, lrm, lrp);
+    eos_wave_speeds<A, M, R, E>(ql, phil, P, llm, llp);
+    eos_wave_speeds<A, M, R, E>(qr, phir, P, lrm, lrp);
     const R s_min = M::min3(llm, lrm, R(0));
     const R s_max = M::max3(llp, lrp, R(0));
     const R den = M::max(s_max - s_min, Consts<R>::den_floor);
@@ -420,7 +487,7 @@ template <class R> __host__ __device__ inline V4<R> add4(const V4<R> &q, const V
 // sanitised primitives of the four cells L-1, L, R, R+1 (step.wgsl:125-145:
 // this is f_plus of cell L and f_minus of cell R; the reference evaluates it
 // once from each side, with identical operands).
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, const V4<R> &qr,
                                            const V4<R> &qrp, R phil, R phir, const Params<R> &P) {
     const V4<R> d = sub4(qr, ql);
@@ -429,16 +496,16 @@ __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, co
     R v2l, v2r;
     const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
     const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
-    return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
+    return hll_flux_v2<A, M, R, E>(a, v2l, phil, b, v2r, phir, P);
 }
 // Same, with the two half slopes (0.5 * limit_slope) already known.
-template <int A, class M, class R>
+template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, const V4<R> &qr,
                                              const V4<R> &hr, R phil, R phir, const Params<R> &P) {
     R v2l, v2r;
     const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
     const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
-    return hll_flux_v2<A, M>(a, v2l, phil, b, v2r, phir, P);
+    return hll_flux_v2<A, M, R, E>(a, v2l, phil, b, v2r, phir, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,7 +640,7 @@ __host__ __device__ inline R bubble_profile(int x, int y, int z, R cx, R cy, R c
 template <class M, class R> struct CellDiag {
     R e_total, broken, kinetic, e_lab, vabs;
 };
-template <class M, class R>
+template <class M, class R, class E = BagEos>
 __host__ __device__ inline CellDiag<M, R> cell_contribution(R phi, R pi, R phixm, R phixp, R phiym, R phiyp,
                                                             R phizm, R phizp, const V4<R> &u,
                                                             const Params<R> &P) {
@@ -591,7 +658,7 @@ __host__ __device__ inline CellDiag<M, R> cell_contribution(R phi, R pi, R phixm
         grad2 = grad2 + d * d;
     }
     const R e_lab = M::max(u.x, P.e_floor);
-    const V4<R> q = eos_prim_from_cons<M>(u, phi, P);
+    const V4<R> q = eos_prim_from_cons<M, R, E>(u, phi, P);
     const R v2 = dot3(q.x, q.y, q.z, q.x, q.y, q.z);
     CellDiag<M, R> c;
     c.e_total = R(0.5) * pi * pi + R(0.5) * grad2 + potential(phi, P) + e_lab;

@@ -35,6 +35,7 @@ namespace {
 int sim_alloc(bs_sim *s) {
     int rc = s->bx.alloc(s->spec.precision, s->spec.grid, s->stream, false);
     if (rc) return rc;
+    s->bx.set_eos(s->spec);
     if (s->spec.kernel == BS_KERNEL_TWO_PASS) {
         const size_t bytes = s->bx.n_cells * 4 * real_size(s->spec.precision);
         hipError_t e = hipMalloc(&s->prim, bytes);

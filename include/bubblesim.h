@@ -84,6 +84,17 @@ enum { BS_SEED_CRITICAL = 0, BS_SEED_CELLS = 1 };
  * identical to the reference's for the same `seed`. */
 enum { BS_NUC_SIMULTANEOUS = 0, BS_NUC_EXPONENTIAL = 1, BS_NUC_UNIFORM_STEPS = 2 };
 
+/* Equation of state behind the swap point of docs/EOS.md:10-34:
+ *   BS_EOS_BAG      : e = 3p, closed-form inversion (eos.wgsl, the default);
+ *   BS_EOS_GAMMA_LAW: p = (eos_gamma - 1) e, c_s^2 = eos_gamma - 1, inverted
+ *                     by a fixed count of Newton steps (docs/EOS.md:86-128);
+ *                     eos_gamma = 4/3 is the bag model.
+ * dof_ratio (docs/EOS.md:65-84) interpolates the radiation constant of the
+ * temperature across the wall, a_broken / a_symmetric; it changes only the
+ * temperature channel of the visualisation.  The batched engine and the
+ * server run the bag model. */
+enum { BS_EOS_BAG = 0, BS_EOS_GAMMA_LAW = 1 };
+
 /* Model (physics.rs:44-68).  seed_kind/seed_value encode SeedSize. */
 typedef struct bs_model {
     float lambda;
@@ -191,8 +202,10 @@ typedef struct bs_spec {
     int32_t kernel;              /* BS_KERNEL_*                                */
     int32_t bubble_at_centre;    /* 1: single bubble at (nx/2, ny/2, nz/2)     */
     uint32_t uniform_max_step;   /* BS_NUC_UNIFORM_STEPS: times in [0, this]   */
-    uint32_t pad0;
+    int32_t eos;                 /* BS_EOS_*                                   */
     uint64_t time_seed;          /* BS_NUC_UNIFORM_STEPS time generator seed   */
+    float eos_gamma;             /* BS_EOS_GAMMA_LAW: 1 < eos_gamma < 2        */
+    float dof_ratio;             /* temperature dof ratio across the wall (> 0) */
 } bs_spec;
 
 /* ------------------------------------------------------------------------ */

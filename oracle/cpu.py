@@ -214,3 +214,14 @@ def potential(params, phi):
 def dpotential(params, phi):
     p, pp = _f32(params)
     return lib().oracle_dpotential(pp, phi)
+
+
+def eos_words(words, eos=0, eos_gamma=4.0 / 3.0, dof_ratio=1.0):
+    """The SimParams words with the oracle's equation-of-state fields set
+    (word 3: kind, words 22 and 23: gamma and the dof ratio)."""
+    w = np.array(words, dtype=np.float32).copy()
+    w.view(np.uint32)[3] = eos
+    w[22] = eos_gamma if eos == 1 else 0.0
+    w[23] = dof_ratio
+    return w
+

@@ -35,6 +35,9 @@ template <class R> struct P4 { R x, y, z, w; };
 template <class R> struct Prm {
     int nx, ny, nz;
     R dx, inv_dx, dt, eta, lambda, eps, phi_b, m2, delta, a_rad, wall_width, p_floor, e_floor, z_max, e_ref, t_ref;
+    int eos;       // 0: bag model (eos.wgsl); 1: gamma-law (docs/EOS.md:86-128)
+    R eos_gamma;   // gamma-law adiabatic index
+    R dof_ratio;   // temperature dof ratio across the wall (docs/EOS.md:65-84)
 };
 
 // WGSL min/max: min(e1,e2) = e2 if e2 < e1 else e1; max = e2 if e1 < e2 else e1
@@ -81,10 +84,32 @@ template <class R> inline R dpotential(const Prm<R> &P, R phi) {
     return phi * (P.m2 - P.delta * phi + P.lambda * phi * phi);
 }
 
-// eos.wgsl
-template <class R> inline R eos_sound_speed2(R, R) { return R(1) / R(3); }
-template <class R> inline R eos_enthalpy_from_pressure(R p, R) { return R(4) * p; }
+// eos.wgsl (bag model) and the gamma-law fluid p = (Gamma - 1) e of
+// docs/EOS.md:86-128: c_s^2 = Gamma - 1, w = Gamma p / (Gamma - 1), inverted
+// by a fixed count of Newton steps on f(p) = p - (Gamma - 1)(E - Z^2 / (E + p))
+// from p0 = (Gamma - 1) E.  (The document's sketch writes de/dp as
+// -(1 + Z^2/s^2) - 1; the derivative of its own e(p) = (s^2 - Z^2)/s - p is
+// Z^2 / s^2, which is what is used here.)
+template <class R> inline R eos_sound_speed2(const Prm<R> &P, R, R) {
+    return P.eos == 1 ? P.eos_gamma - R(1) : R(1) / R(3);
+}
+template <class R> inline R eos_enthalpy_from_pressure(const Prm<R> &P, R p, R) {
+    return P.eos == 1 ? P.eos_gamma * p / (P.eos_gamma - R(1)) : R(4) * p;
+}
 template <class R> inline R eos_pressure_from_conserved(const Prm<R> &P, R e_lab, R z2, R) {
+    if (P.eos == 1) {
+        const R gm1 = P.eos_gamma - R(1);
+        R p = gm1 * e_lab;
+        for (int it = 0; it < 8; ++it) {
+            const R s = e_lab + p;
+            const R rs = R(1) / s;
+            const R zs = z2 * rs;
+            const R f = p - gm1 * (e_lab - zs);
+            const R df = R(1) - gm1 * (zs * rs);
+            p = p - f / df;
+        }
+        return wmax(p, P.p_floor);
+    }
     const R disc = wmax(R(4) * e_lab * e_lab - R(3) * z2, R(0));
     const R p = (e_lab * e_lab - z2) / (e_lab + std::sqrt(disc));
     return wmax(p, P.p_floor);
@@ -112,7 +137,7 @@ template <class R> inline P4<R> eos_prim_from_cons(const Prm<R> &P, const P4<R> 
 }
 template <class R> inline P4<R> eos_cons_from_prim(const Prm<R> &P, const P4<R> &q, R phi) {
     const R v2 = wmin(wdot(q, q), P.z_max * P.z_max);
-    const R w_gam2 = eos_enthalpy_from_pressure(q.w, phi) / (R(1) - v2);
+    const R w_gam2 = eos_enthalpy_from_pressure(P, q.w, phi) / (R(1) - v2);
     return {w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 template <class R> inline R axis_comp(const P4<R> &v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
@@ -128,7 +153,7 @@ template <class R> inline P4<R> eos_flux(const P4<R> &q, const P4<R> &u, int axi
 template <class R> inline void eos_wave_speeds(const Prm<R> &P, const P4<R> &q, R phi, int axis, R &lo, R &hi) {
     const R v2 = wmin(wdot(q, q), P.z_max * P.z_max);
     const R vd = axis_comp(q, axis);
-    const R cs2 = eos_sound_speed2(q.w, phi);
+    const R cs2 = eos_sound_speed2(P, q.w, phi);
     const R cs = std::sqrt(cs2);
     const R denom = R(1) - v2 * cs2;
     const R rad = wmax((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
@@ -406,6 +431,12 @@ template <class R> Prm<R> make_prm(const float *p32) {
     P.lambda = p32[8]; P.eps = p32[9]; P.phi_b = p32[10]; P.m2 = p32[11];
     P.delta = p32[12]; P.a_rad = p32[13]; P.wall_width = p32[14]; P.p_floor = p32[15];
     P.e_floor = p32[16]; P.z_max = p32[17]; P.e_ref = p32[18]; P.t_ref = p32[19];
+    // the block's padding words carry the equation of state for the oracle:
+    // word 3 the kind, words 22 and 23 gamma and the dof ratio (zero in the
+    // reference's block: the bag model, dof ratio 1)
+    P.eos = (int)u[3];
+    P.eos_gamma = P.eos == 1 ? (R)p32[22] : (R)(4.0f / 3.0f);
+    P.dof_ratio = p32[23] != 0.0f ? (R)p32[23] : R(1);
     return P;
 }
 
@@ -434,8 +465,15 @@ template <class R> Lattice<R> *make_lattice(const float *params, const float *ev
 }
 
 // eos_temperature (eos.wgsl:65-69): T = (3 max(p, p_floor) / a)^(1/4)
-template <class R> inline R eos_temperature(const Prm<R> &P, R p, R /*phi*/) {
-    const R x = R(3) * wmax(p, P.p_floor) / P.a_rad;
+template <class R> inline R eos_temperature(const Prm<R> &P, R p, R phi) {
+    R a = P.a_rad;
+    if (P.dof_ratio != R(1)) {
+        // docs/EOS.md:70-76: a(phi) = a (1 + (dof_ratio - 1) h^2 (3 - 2h))
+        const R h = wmin(wmax(phi / P.phi_b, R(0)), R(1));
+        const R smooth = h * h * (R(3) - R(2) * h);
+        a = P.a_rad * (R(1) + (P.dof_ratio - R(1)) * smooth);
+    }
+    const R x = R(3) * wmax(p, P.p_floor) / a;
     return std::sqrt(std::sqrt(x));
 }
 

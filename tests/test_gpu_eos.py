@@ -1,0 +1,129 @@
+"""The equation-of-state swap point on the GPU (docs/EOS.md).
+
+* ExactMath kernels with the gamma-law EOS (fused and two-pass, f32 and f64)
+  equal the oracle's restatement bit for bit, nucleation included;
+* FastMath gamma-law stays within twice the exact f32 path's distance to f64;
+* a box at rest stays exactly at rest; Gamma = 4/3 tracks the bag model;
+* the dof-ratio temperature channel equals the oracle's, bit for bit in half;
+* the batched engine and the server refuse a non-bag EOS.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GAMMA = 1.45
+
+
+def _sched(bs, grid):
+    r = bs.Model().seed_radius()
+    return np.array([(0.0, grid[0] / 2, grid[1] / 2, grid[2] / 2, r), (0.6, 3.0, 5.0, 7.5, r)], dtype=np.float32)
+
+
+def _pair(bs, orc, grid, precision, math, kernel, gamma=GAMMA, dof=1.0):
+    model = bs.Model()
+    spec = bs.make_spec(grid=grid, model=model, precision=precision, math=math, kernel=kernel,
+                        nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, eos=bs.EOS_GAMMA_LAW, eos_gamma=gamma,
+                        dof_ratio=dof)
+    sim = bs.Simulation(spec)
+    sched = _sched(bs, grid)
+    sim.reset(sched)
+    words = orc.eos_words(bs.sim_params_words(model, grid), eos=1, eos_gamma=gamma, dof_ratio=dof)
+    o = orc.OracleSim(1 if precision == bs.PRECISION_F64 else 0, words, sched)
+    return sim, o
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gamma_law_exact_matches_oracle_bitwise(bs, orc, gpu_available, precision, kernel):
+    grid = (40, 24, 20)
+    print(f"case precision={precision} kernel={kernel}", flush=True)
+    sim, o = _pair(bs, orc, grid, precision, bs.MATH_EXACT, kernel)
+    for chunk in (1, 2, 3):
+        sim.advance(chunk)
+        o.advance(chunk)
+        gf, gu = sim.download()
+        of, ou = o.state()
+        assert np.array_equal(gf, of), f"field differs after {sim.steps} steps: max {np.abs(gf - of).max()}"
+        assert np.array_equal(gu, ou), f"fluid differs after {sim.steps} steps: max {np.abs(gu - ou).max()}"
+    d = sim.diagnostics()
+    sums = o.diagnostics_sums()
+    n = sim.cell_count()
+    assert d.mean_energy == pytest.approx(sums[0] / n, rel=1e-12)
+    assert d.mean_kinetic == pytest.approx(sums[2] / n, rel=1e-12, abs=1e-30)
+    sim.close()
+    o.close()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def test_gamma_law_fast_close_to_exact(bs, orc, gpu_available):
+    grid = (32, 32, 32)
+    out = {}
+    for name, prec, math in (("fast", 0, bs.MATH_FAST), ("exact", 0, bs.MATH_EXACT), ("f64", 1, bs.MATH_EXACT)):
+        sim, o = _pair(bs, orc, grid, prec, math, bs.KERNEL_FUSED)
+        sim.advance(20)
+        out[name] = [x.astype(np.float64) for x in sim.download()]
+        sim.close()
+        o.close()
+    for k in range(2):
+        ef = _rel(out["fast"][k], out["f64"][k])
+        ee = _rel(out["exact"][k], out["f64"][k])
+        print(f"block {k}: fast-vs-f64 {ef:.3e} exact-f32-vs-f64 {ee:.3e}", flush=True)
+        assert ef < 1e-5 and ef <= 2.0 * ee + 1e-7
+
+
+def test_gamma_law_rest_is_a_fixed_point(bs, gpu_available):
+    for math in (bs.MATH_FAST, bs.MATH_EXACT):
+        spec = bs.make_spec(grid=24, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, math=math, eos=bs.EOS_GAMMA_LAW,
+                            eos_gamma=GAMMA)
+        sim = bs.Simulation(spec)
+        sim.reset(np.zeros((0, 5), np.float32))
+        f0, u0 = sim.download()
+        sim.advance(10)
+        f1, u1 = sim.download()
+        assert np.array_equal(f0, f1) and np.array_equal(u0, u1)
+        assert sim.diagnostics().max_velocity == 0.0
+        sim.close()
+
+
+def test_gamma_four_thirds_tracks_the_bag_model(bs, gpu_available):
+    grid = 48
+    res = []
+    for eos in (bs.EOS_BAG, bs.EOS_GAMMA_LAW):
+        spec = bs.make_spec(grid=grid, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, eos=eos, eos_gamma=4.0 / 3.0,
+                            math=bs.MATH_EXACT)
+        sim = bs.Simulation(spec)
+        sim.advance(60)
+        d = sim.diagnostics()
+        res.append((d.mean_energy, d.broken_fraction, d.mean_kinetic, d.max_velocity))
+        sim.close()
+    for a, b in zip(*res):
+        assert a == pytest.approx(b, rel=2e-5, abs=1e-12)
+
+
+def test_dof_ratio_texels_equal_oracle(bs, orc, gpu_available):
+    grid = (32, 24, 16)
+    sim, o = _pair(bs, orc, grid, 0, bs.MATH_FAST, bs.KERNEL_FUSED, dof=0.7)
+    sim.advance(5)
+    sim.record_visualisation()
+    tex = sim.download_visualisation()
+    f, u = sim.download()
+    o.set_state(f, u)
+    want = o.vis_channels().astype(np.float16).reshape(tex.shape)
+    assert np.array_equal(tex.view(np.uint16), want.view(np.uint16))
+    sim.close()
+    o.close()
+
+
+def test_batch_and_server_refuse_other_eos(bs, gpu_available):
+    spec = bs.make_spec(grid=16, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, eos=bs.EOS_GAMMA_LAW, eos_gamma=1.5)
+    with pytest.raises(bs.BubbleSimError) as e:
+        bs.Batch([spec])
+    assert e.value.code == bs.BS_ERR_INVALID
+    srv = bs.Server(budget_bytes=1 << 30, chunk_steps=5)
+    with pytest.raises(bs.BubbleSimError) as e:
+        srv.submit(spec, 10)
+    assert e.value.code == bs.BS_ERR_INVALID
+    srv.close()
