@@ -38,11 +38,11 @@ def sweep_eta(bs, seed=7):
     return math.exp(math.log(0.02) + u * (math.log(1.0) - math.log(0.02)))
 
 
-def headline_spec(bs, n=512):
+def headline_spec(bs, n=512, eos=0, eos_gamma=4.0 / 3.0):
     model = bs.Model(eta=sweep_eta(bs))
     return bs.make_spec(grid=n, model=model, nucleation=bs.NUC_UNIFORM_STEPS, bubbles=32, seed=1,
                         uniform_max_step=200, time_seed=11, precision=bs.PRECISION_F32, math=bs.MATH_FAST,
-                        kernel=bs.KERNEL_FUSED)
+                        kernel=bs.KERNEL_FUSED, eos=eos, eos_gamma=eos_gamma)
 
 
 def cpu_baseline(bs, budget_s=12.0):
@@ -207,6 +207,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--eos", choices=["bag", "gamma-law"], default="bag",
+                    help="equation of state of the stepped box (the headline is the bag model)")
+    ap.add_argument("--eos-gamma", type=float, default=1.45, help="gamma-law adiabatic index")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 1 GiB device copies (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -222,7 +225,8 @@ def main():
         print("no HIP device", file=sys.stderr)
         return 1
     n = args.grid
-    spec = headline_spec(bs, n)
+    eos = 1 if args.eos == "gamma-law" else 0
+    spec = headline_spec(bs, n, eos, args.eos_gamma if eos else 4.0 / 3.0)
     log(f"bench: {n}^3 f32 fused, eta={spec.model.eta:.4f}, device {bs.device_name()}")
     sim = bs.Simulation(spec)
     cells = sim.cell_count()
@@ -246,7 +250,7 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_fused_stage_512.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and n == 512 and not eos:  # measured on the headline workload only
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
     sim.close()
@@ -268,7 +272,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"single periodic {n}^3 box, 32 bubbles (seeded centres, steps U[0,200]), "
                                f"eta log-U[0.02,1] draw = {spec.model.eta:.4f}, SSP-RK3, fused stage kernel",
-                   "grid": n, "bubbles": 32, "precision": "f32", "math": "fast"},
+                   "grid": n, "bubbles": 32, "precision": "f32", "math": "fast",
+                   "eos": args.eos if not eos else f"gamma-law {args.eos_gamma}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
