@@ -243,18 +243,25 @@ __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params
     R zx = u.y, zy = u.z, zz = u.w;
     const R z2raw = dot3(zx, zy, zz, zx, zy, zz);
     const R zcap = P.z_max * e_lab;
-    bool clamp;
+    R z2;
     if constexpr (M::fast) {
-        clamp = z2raw > zcap * zcap;  // |Z| > zcap without the square root (zcap > 0)
-    } else {
-        clamp = M::sqrt(z2raw) > zcap;
-    }
-    if (clamp) {
-        const R zlen = M::sqrt(z2raw);
-        const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
+        // |Z| > zcap without the square root (zcap > 0); the rescale is
+        // branch-free, by 1 when nothing is clamped, and |Z|^2 after it is
+        // zcap^2 itself
+        const R zcap2 = zcap * zcap;
+        const bool clamp = z2raw > zcap2;
+        const R f = clamp ? zcap * M::inv_sqrt(z2raw) : R(1);
         zx = zx * f; zy = zy * f; zz = zz * f;
+        z2 = clamp ? zcap2 : z2raw;
+    } else {
+        if (M::sqrt(z2raw) > zcap) {
+            const R zlen = M::sqrt(z2raw);
+            const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
+            zx = zx * f; zy = zy * f; zz = zz * f;
+        }
+        z2 = dot3(zx, zy, zz, zx, zy, zz);
     }
-    const R p = E::template pressure_from_conserved<M>(e_lab, dot3(zx, zy, zz, zx, zy, zz), phi, P);
+    const R p = E::template pressure_from_conserved<M>(e_lab, z2, phi, P);
     const R s = e_lab + p;
     const R sm = M::fast ? s : M::max(s, Consts<R>::tiny);  // s >= e_floor + p_floor > 0
     return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
