@@ -35,6 +35,14 @@ __host__ __device__ inline void stage_slot_ids(int cur, int s, int &in, int &out
     out = s == 1 ? c : b;
 }
 
+// The value that lane l - 1 of the wave holds (0 in lane 0): DPP wave_shr:1.
+__device__ __forceinline__ float lane_from_left(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ V4<float> lane_from_left(const V4<float> &v) {
+    return V4<float>{lane_from_left(v.x), lane_from_left(v.y), lane_from_left(v.z), lane_from_left(v.w)};
+}
+
 template <class R> struct RK3c;
 template <> struct RK3c<float> {
     // RK3 (sim.rs:135)
@@ -253,33 +261,96 @@ template <class R> struct VecT;
 template <> struct VecT<float> { typedef f4v v4; typedef f2v v2; };
 template <> struct VecT<double> { typedef d4v v4; typedef d2v v2; };
 
+#ifndef BS_GLOBAL_PLANE_ACCESS
+// Plane access through a raw buffer resource: the plane's base and size sit
+// in four scalar registers (rebuilt per plane by a few scalar instructions)
+// and each lane supplies only a 32-bit byte offset that is fixed for the
+// whole z-march, so no vector instruction is spent on addresses inside the
+// loop.  Offsets at or past the plane's size read zero and drop stores, the
+// hardware's range check.  Cache policy 2 is the non-temporal hint.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data, no swizzle
+constexpr int kCacheDefault = 0, kCacheStream = 2;
+
+template <class E> struct PlaneRef {
+    __amdgpu_buffer_rsrc_t r;
+};
+template <class E> __device__ __forceinline__ PlaneRef<E> plane_ref(const E *base, unsigned elems) {
+    return PlaneRef<E>{__builtin_amdgcn_make_buffer_rsrc((void *)uniform_ptr(base), 0, (int)(elems * (unsigned)sizeof(E)),
+                                                         kRawBufferWord3)};
+}
+template <int AUX> __device__ __forceinline__ u32x4 buf_load16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, AUX);
+}
+template <int AUX, class E> __device__ __forceinline__ E buf_load(const PlaneRef<E> &p, unsigned idx) {
+    const unsigned off = idx * (unsigned)sizeof(E);
+    if constexpr (sizeof(E) == 8) {
+        return __builtin_bit_cast(E, __builtin_amdgcn_raw_buffer_load_b64(p.r, (int)off, 0, AUX));
+    } else if constexpr (sizeof(E) == 16) {
+        return __builtin_bit_cast(E, buf_load16<AUX>(p.r, off));
+    } else {
+        static_assert(sizeof(E) == 32, "plane elements are 8, 16 or 32 bytes");
+        struct W { u32x4 a, b; };
+        return __builtin_bit_cast(E, W{buf_load16<AUX>(p.r, off), buf_load16<AUX>(p.r, off + 16)});
+    }
+}
+template <int AUX, class E> __device__ __forceinline__ void buf_store(const PlaneRef<E> &p, unsigned idx, const E &v) {
+    const unsigned off = idx * (unsigned)sizeof(E);
+    if constexpr (sizeof(E) == 8) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), p.r, (int)off, 0, AUX);
+    } else if constexpr (sizeof(E) == 16) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), p.r, (int)off, 0, AUX);
+    } else {
+        static_assert(sizeof(E) == 32, "plane elements are 8, 16 or 32 bytes");
+        struct W { u32x4 a, b; };
+        const W w = __builtin_bit_cast(W, v);
+        __builtin_amdgcn_raw_buffer_store_b128(w.a, p.r, (int)off, 0, AUX);
+        __builtin_amdgcn_raw_buffer_store_b128(w.b, p.r, (int)(off + 16), 0, AUX);
+    }
+}
+template <class E> __device__ __forceinline__ E plane_load(const PlaneRef<E> &p, unsigned idx) {
+    return buf_load<kCacheDefault>(p, idx);
+}
+template <class E> __device__ __forceinline__ E stream_load(const PlaneRef<E> &p, unsigned idx) {
+    return buf_load<kCacheStream>(p, idx);
+}
+template <class E> __device__ __forceinline__ void stream_store(const PlaneRef<E> &p, unsigned idx, const E &v) {
+    buf_store<kCacheStream>(p, idx, v);
+}
+#else
+template <class E> struct PlaneRef {
+    const E *base;
+};
+template <class E> __device__ __forceinline__ PlaneRef<E> plane_ref(const E *base, unsigned) { return PlaneRef<E>{base}; }
 template <class T, class E> __device__ __forceinline__ global_t<T> *elem_ptr(const E *base, unsigned idx) {
     return reinterpret_cast<global_t<T> *>(uniform_ptr(base) + idx * (unsigned)sizeof(E));
 }
-template <class R> __device__ __forceinline__ V4<R> plane_load(const V4<R> *base, unsigned idx) {
-    const typename VecT<R>::v4 v = *elem_ptr<const typename VecT<R>::v4>(base, idx);
+template <class R> __device__ __forceinline__ V4<R> plane_load(const PlaneRef<V4<R>> &p, unsigned idx) {
+    const typename VecT<R>::v4 v = *elem_ptr<const typename VecT<R>::v4>(p.base, idx);
     return V4<R>{v.x, v.y, v.z, v.w};
 }
-template <class R> __device__ __forceinline__ V2<R> plane_load(const V2<R> *base, unsigned idx) {
-    const typename VecT<R>::v2 v = *elem_ptr<const typename VecT<R>::v2>(base, idx);
+template <class R> __device__ __forceinline__ V2<R> plane_load(const PlaneRef<V2<R>> &p, unsigned idx) {
+    const typename VecT<R>::v2 v = *elem_ptr<const typename VecT<R>::v2>(p.base, idx);
     return V2<R>{v.x, v.y};
 }
-template <class R> __device__ __forceinline__ V4<R> stream_load(const V4<R> *base, unsigned idx) {
-    const typename VecT<R>::v4 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v4>(base, idx));
+template <class R> __device__ __forceinline__ V4<R> stream_load(const PlaneRef<V4<R>> &p, unsigned idx) {
+    const typename VecT<R>::v4 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v4>(p.base, idx));
     return V4<R>{v.x, v.y, v.z, v.w};
 }
-template <class R> __device__ __forceinline__ V2<R> stream_load(const V2<R> *base, unsigned idx) {
-    const typename VecT<R>::v2 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v2>(base, idx));
+template <class R> __device__ __forceinline__ V2<R> stream_load(const PlaneRef<V2<R>> &p, unsigned idx) {
+    const typename VecT<R>::v2 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v2>(p.base, idx));
     return V2<R>{v.x, v.y};
 }
-template <class R> __device__ __forceinline__ void stream_store(V4<R> *base, unsigned idx, const V4<R> &v) {
+template <class R> __device__ __forceinline__ void stream_store(const PlaneRef<V4<R>> &p, unsigned idx, const V4<R> &v) {
     typename VecT<R>::v4 t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v4>(base, idx));
+    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v4>(p.base, idx));
 }
-template <class R> __device__ __forceinline__ void stream_store(V2<R> *base, unsigned idx, const V2<R> &v) {
+template <class R> __device__ __forceinline__ void stream_store(const PlaneRef<V2<R>> &p, unsigned idx, const V2<R> &v) {
     typename VecT<R>::v2 t = {v.x, v.y};
-    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(base, idx));
+    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(p.base, idx));
 }
+#endif
 
 template <int TX, int TY, class M, class R, bool S0, class E>
 __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
@@ -337,7 +408,13 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // code path instead of one per axis.  The swap commutes with every
     // operation of the face (the sums of squares add the same two products
     // in either order), so the result is the y face's, bit for bit.
-    constexpr int NTAIL = TX + TY;
+    // FastMath shares each cell's x half slope with the face on its left:
+    // the cell forms q - h (the right state of its left face) and q + h (the
+    // left state of its right face), and the lane to its right takes the
+    // latter over DPP, so each x slope is formed once instead of twice.  The
+    // faces at x = 0 have no lane on their left and join the far-edge round.
+    constexpr bool XSHARE = M::fast;
+    constexpr int NTAIL = TX + TY + (XSHARE ? TY : 0);
     const bool tail = tid < NTAIL;
 
     // sanitised primitives of a cell.  eos_prim_from_cons already floors p
@@ -351,68 +428,79 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
     // (phi of plane z-1, U_in of planes z, z+1 and the primitives of z, z+1
     // stay in registers, so every input plane is read from memory once per
-    // column)
-    V2<R> f0, f1, ftmp;
-    V4<R> u0, u1, utmp;
-    R ph_m;
-    ftmp = plane_load(field_in + wrapi(zb - 2, P.nz) * plane, col);
-    utmp = plane_load(fluid_in + wrapi(zb - 2, P.nz) * plane, col);
+    // column).  (Unrolling the march by three so that the ring shifts
+    // become register renames needs more than the 96 VGPRs of 5 waves per
+    // SIMD: it spills, or at 4 waves per SIMD runs 6 % slower.)
+    V2<R> fr[3], ftmp;  // field of planes z, z+1, z+2
+    V4<R> ur[3], utmp;  // U_in of the same planes
+    V4<R> qr[3];        // their primitives
+    V4<R> hr[2];        // z half slopes of planes z, z+1
+    V4<R> fzr[2];       // z faces z-1/2 and z+1/2
+    R ph_m;             // phi of plane z-1
+    auto field_at = [&](const V2<R> *a, int zz) { return plane_ref(a + (int64_t)zz * plane, (unsigned)plane); };
+    auto fluid_at = [&](const V4<R> *a, int zz) { return plane_ref(a + (int64_t)zz * plane, (unsigned)plane); };
+    ftmp = plane_load(field_at(field_in, wrapi(zb - 2, P.nz)), col);
+    utmp = plane_load(fluid_at(fluid_in, wrapi(zb - 2, P.nz)), col);
     const V4<R> qa = prim_of(ftmp, utmp);
-    ftmp = plane_load(field_in + wrapi(zb - 1, P.nz) * plane, col);
-    utmp = plane_load(fluid_in + wrapi(zb - 1, P.nz) * plane, col);
+    ftmp = plane_load(field_at(field_in, wrapi(zb - 1, P.nz)), col);
+    utmp = plane_load(fluid_at(fluid_in, wrapi(zb - 1, P.nz)), col);
     const V4<R> qb = prim_of(ftmp, utmp);
     ph_m = ftmp.x;
-    f0 = plane_load(field_in + wrapi(zb, P.nz) * plane, col);
-    u0 = plane_load(fluid_in + wrapi(zb, P.nz) * plane, col);
-    V4<R> qz = prim_of(f0, u0);
-    f1 = plane_load(field_in + wrapi(zb + 1, P.nz) * plane, col);
-    u1 = plane_load(fluid_in + wrapi(zb + 1, P.nz) * plane, col);
-    V4<R> qz1 = prim_of(f1, u1);
-    const V4<R> h_b = half_slope4<M>(sub4(qb, qa), sub4(qz, qb));
-    V4<R> hz = half_slope4<M>(sub4(qz, qb), sub4(qz1, qz));
-    V4<R> fzm = face_flux_h<2, M, R, E>(qb, h_b, qz, hz, ph_m, f0.x, P);
+    fr[0] = plane_load(field_at(field_in, wrapi(zb, P.nz)), col);
+    ur[0] = plane_load(fluid_at(fluid_in, wrapi(zb, P.nz)), col);
+    qr[0] = prim_of(fr[0], ur[0]);
+    fr[1] = plane_load(field_at(field_in, wrapi(zb + 1, P.nz)), col);
+    ur[1] = plane_load(fluid_at(fluid_in, wrapi(zb + 1, P.nz)), col);
+    qr[1] = prim_of(fr[1], ur[1]);
+    const V4<R> h_b = half_slope4<M>(sub4(qb, qa), sub4(qr[0], qb));
+    hr[0] = half_slope4<M>(sub4(qr[0], qb), sub4(qr[1], qr[0]));
+    fzr[0] = face_flux_h<2, M, R, E>(qb, h_b, qr[0], hr[0], ph_m, fr[0].x, P);
 
     V4<R> *qflat = &L.q[0][0];
     R *phiflat = &L.phi[0][0];
     // halo cells are recovered two planes ahead, together with the column
     // load of the same plane, and wait in a thread-private LDS ring
     auto halo_ahead = [&](int zz, int slot) {
-        const int64_t zo = (int64_t)zz * plane;
+        const PlaneRef<V2<R>> fp = field_at(field_in, zz);
+        const PlaneRef<V4<R>> up = fluid_at(fluid_in, zz);
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
                 const int h = tid + k * NT;
-                const V2<R> fh = plane_load(field_in + zo, hoff[k]);
-                const V4<R> uh = plane_load(fluid_in + zo, hoff[k]);
+                const V2<R> fh = plane_load(fp, hoff[k]);
+                const V4<R> uh = plane_load(up, hoff[k]);
                 L.hq[slot][h] = prim_of(fh, uh);
                 L.hph[slot][h] = fh.x;
             }
         }
     };
-    halo_ahead(zb, zb % 3);
-    if (zb + 1 < ze) halo_ahead(wrapi(zb + 1, P.nz), (zb + 1) % 3);
+    halo_ahead(zb, 0);
+    if (zb + 1 < ze) halo_ahead(wrapi(zb + 1, P.nz), 1);
 
-    for (int z = zb; z < ze; ++z) {
+    // hs = (z - zb) mod 3 is the halo ring slot of plane z
+    for (int z = zb, hs = 0; z < ze; ++z) {
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
-        const int64_t zoff = (int64_t)z * plane;
+        const V2<R> f0 = fr[0], f1 = fr[1];
+        const V4<R> qz = qr[0], qz1 = qr[1], hz = hr[0], fzm = fzr[0];
         // the update's operand U^n(z) of this cell, loaded first so that it
         // arrives while the faces are computed (U_in(z) is in the ring)
         const R pi0 = f0.y;
-        const V4<R> uin = u0;
+        const V4<R> uin = ur[0];
         V2<R> fnn;
         V4<R> un;
         if constexpr (S0) {
             fnn = V2<R>{f0.x, pi0};  // U_in is U^n at stage 0 (same slot)
             un = uin;
         } else {
-            fnn = stream_load(field_n + zoff, col);
-            un = stream_load(fluid_n + zoff, col);
+            fnn = stream_load(field_at(field_n, z), col);
+            un = stream_load(fluid_at(fluid_n, z), col);
         }
         // (a) plane z+2 of this column (for the +z face)
-        const V2<R> f2 = plane_load(field_in + zp2 * plane, col);
-        const V4<R> u2 = plane_load(fluid_in + zp2 * plane, col);
-        const V4<R> q2 = prim_of(f2, u2);
-        if (z + 2 < ze) halo_ahead(zp2, (z + 2) % 3);
+        fr[2] = plane_load(field_at(field_in, zp2), col);
+        ur[2] = plane_load(fluid_at(fluid_in, zp2), col);
+        const V4<R> q2 = prim_of(fr[2], ur[2]);
+        qr[2] = q2;
+        if (z + 2 < ze) halo_ahead(zp2, hs == 0 ? 2 : hs - 1);
 
         // (b) plane z into LDS: interior from registers, halo from the ring
         L.q[ty + 2][tx + 2] = qz;
@@ -421,26 +509,39 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
                 const int h = tid + k * NT;
-                qflat[hlds[k]] = L.hq[z % 3][h];
-                if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[z % 3][h];
+                qflat[hlds[k]] = L.hq[hs][h];
+                if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[hs][h];
             }
         }
         __syncthreads();
 
         // (c) x and y faces, slopes formed inside each face from four cells
-        L.fx[ty][tx] = face_flux<0, M, R, E>(L.q[ty + 2][tx], L.q[ty + 2][tx + 1], L.q[ty + 2][tx + 2], L.q[ty + 2][tx + 3],
-                                       L.phi[ty + 1][tx], L.phi[ty + 1][tx + 1], P);
-        L.fy[ty][tx] = face_flux<1, M, R, E>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], L.q[ty + 2][tx + 2], L.q[ty + 3][tx + 2],
-                                       L.phi[ty][tx + 1], L.phi[ty + 1][tx + 1], P);
+        if constexpr (XSHARE) {
+            const V4<R> h = half_slope4<M>(sub4(qz, L.q[ty + 2][tx + 1]), sub4(L.q[ty + 2][tx + 3], qz));
+            R v2r, v2l;
+            const V4<R> right = sanitise_v2<M>(add4(qz, h), P, v2r);  // left state of the face at x + 1/2
+            const V4<R> b = sanitise_v2<M>(sub4(qz, h), P, v2l);      // right state of the face at x - 1/2
+            const V4<R> a = lane_from_left(right);
+            const V4<R> fl = hll_flux_v2<0, M, R, E>(a, lane_from_left(v2r), L.phi[ty + 1][tx], b, v2l, f0.x, P);
+            if (tx != 0) L.fx[ty][tx] = fl;
+        } else {
+            L.fx[ty][tx] = face_flux<0, M, R, E>(L.q[ty + 2][tx], L.q[ty + 2][tx + 1], qz, L.q[ty + 2][tx + 3],
+                                                 L.phi[ty + 1][tx], f0.x, P);
+        }
+        L.fy[ty][tx] = face_flux<1, M, R, E>(L.q[ty][tx + 2], L.q[ty + 1][tx + 2], qz, L.q[ty + 3][tx + 2],
+                                             L.phi[ty][tx + 1], f0.x, P);
         if (tail) {
-            // far-edge face `tid`: x faces x = TX of rows 0..TY-1, then y faces
-            // y = TY of columns 0..TX-1 (addresses formed here, in the one
-            // wave that runs this round, to keep them out of registers)
-            const bool ya = tid >= TY;
-            const int c = ya ? tid - TY : tid;
-            const int q0 = ya ? TY * (TX + 4) + (c + 2) : (c + 2) * (TX + 4) + TX;
+            // far-edge face `tid`: x faces x = TX of rows 0..TY-1, y faces
+            // y = TY of columns 0..TX-1, then (XSHARE) x faces x = 0 of rows
+            // 0..TY-1 (addresses formed here, in the wave that runs this
+            // round, to keep them out of registers)
+            const bool ya = tid >= TY && tid < TY + TX;
+            const bool xl = tid >= TY + TX;
+            const int c = ya ? tid - TY : (xl ? tid - (TY + TX) : tid);
+            const int xe = xl ? 0 : TX;
+            const int q0 = ya ? TY * (TX + 4) + (c + 2) : (c + 2) * (TX + 4) + xe;
             const int qs = ya ? TX + 4 : 1;
-            const int p0 = ya ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + TX;
+            const int p0 = ya ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + xe;
             const int ps = ya ? TX + 2 : 1;
             V4<R> c0 = qflat[q0], c1 = qflat[q0 + qs], c2 = qflat[q0 + 2 * qs], c3 = qflat[q0 + 3 * qs];
             if (ya) {
@@ -450,12 +551,14 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             if (ya) {
                 L.fy[TY][c] = swap_yz(ft);
             } else {
-                L.fx[c][TX] = ft;
+                L.fx[c][xe] = ft;
             }
         }
         // own +z face
         const V4<R> hz1 = half_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
         const V4<R> fzp = face_flux_h<2, M, R, E>(qz, hz, qz1, hz1, f0.x, f1.x, P);
+        hr[1] = hz1;
+        fzr[1] = fzp;
 
         // field stencil from LDS / registers (step.wgsl:118-122)
         const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
@@ -499,14 +602,19 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             V4<R> fo;
             stage_update<M, R, S0>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, fnn.x, fnn.y, un, a0, a1, dt, P,
                                    phi_o, pi_o, fo);
-            stream_store(field_out + zoff, col, V2<R>{phi_o, pi_o});
-            stream_store(fluid_out + zoff, col, fo);
+            stream_store(field_at(field_out, z), col, V2<R>{phi_o, pi_o});
+            stream_store(fluid_at(fluid_out, z), col, fo);
         }
 
-        // (e) rotate the column ring
-        qz = qz1; qz1 = q2; hz = hz1; fzm = fzp;
+
+        // (e) advance the column ring
         ph_m = f0.x;
-        f0 = f1; f1 = f2; u0 = u1; u1 = u2;
+        fr[0] = fr[1]; fr[1] = fr[2];
+        ur[0] = ur[1]; ur[1] = ur[2];
+        qr[0] = qr[1]; qr[1] = qr[2];
+        hr[0] = hr[1];
+        fzr[0] = fzr[1];
+        hs = hs == 2 ? 0 : hs + 1;
     }
 }
 
@@ -517,8 +625,11 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 template <class M> struct Occupancy {
     static constexpr int waves = 1;
 };
+#ifndef BS_FAST_WAVES_PER_EU
+#define BS_FAST_WAVES_PER_EU 5
+#endif
 template <> struct Occupancy<FastMathF32> {
-    static constexpr int waves = 5;
+    static constexpr int waves = BS_FAST_WAVES_PER_EU;
 };
 
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with

@@ -337,6 +337,19 @@ int spec_validate(const bs_spec &s, std::string &err) {
         err = "precision must be BS_PRECISION_F32 or BS_PRECISION_F64";
         return BS_ERR_INVALID;
     }
+    {
+        // the stage kernels address one x-y plane of fluid states (16 B per
+        // cell in f32, 32 B in f64) with 32-bit byte offsets
+        const uint64_t plane_cells = (uint64_t)s.grid[0] * s.grid[1];
+        const uint64_t cap = s.precision == BS_PRECISION_F64 ? (1ull << 27) : (1ull << 28);
+        if (plane_cells >= cap) {
+            char buf[200];
+            std::snprintf(buf, sizeof buf, "an x-y plane of %llu cells is too large; nx * ny must stay below %llu",
+                          (unsigned long long)plane_cells, (unsigned long long)cap);
+            err = buf;
+            return BS_ERR_INVALID;
+        }
+    }
     if (s.math != BS_MATH_FAST && s.math != BS_MATH_EXACT) {
         err = "math must be BS_MATH_FAST or BS_MATH_EXACT";
         return BS_ERR_INVALID;

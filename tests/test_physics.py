@@ -137,6 +137,14 @@ def test_config_validate(bs):
     with pytest.raises(bs.BubbleSimError, match="grid axis 1 is 7"):
         bs.validate_spec(bs.make_spec(grid=(8, 7, 8)))
     bs.validate_spec(bs.make_spec(grid=(8, 8, 8)))
+    # an x-y plane is addressed with 32-bit byte offsets: 2^28 cells of
+    # 16 B (f32) or 2^27 cells of 32 B (f64) is one too many
+    with pytest.raises(bs.BubbleSimError, match="x-y plane of 268435456 cells"):
+        bs.validate_spec(bs.make_spec(grid=(16384, 16384, 8)))
+    bs.validate_spec(bs.make_spec(grid=(16384, 16383, 8)))
+    with pytest.raises(bs.BubbleSimError, match="nx \\* ny must stay below 134217728"):
+        bs.validate_spec(bs.make_spec(grid=(16384, 8192, 8), precision=bs.PRECISION_F64))
+    bs.validate_spec(bs.make_spec(grid=(16384, 8191, 8), precision=bs.PRECISION_F64))
 
 
 def test_grid_accepts_one_or_three_values(bs):
