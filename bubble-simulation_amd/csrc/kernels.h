@@ -460,19 +460,32 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     R *phiflat = &L.phi[0][0];
     // halo cells are recovered two planes ahead, together with the column
     // load of the same plane, and wait in a thread-private LDS ring
-    auto halo_ahead = [&](int zz, int slot) {
+    V2<R> hfr[HPT];  // raw halo states of the plane being fetched
+    V4<R> hur[HPT];
+    auto halo_load = [&](int zz) {
         const PlaneRef<V2<R>> fp = field_at(field_in, zz);
         const PlaneRef<V4<R>> up = fluid_at(fluid_in, zz);
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                const int h = tid + k * NT;
-                const V2<R> fh = plane_load(fp, hoff[k]);
-                const V4<R> uh = plane_load(up, hoff[k]);
-                L.hq[slot][h] = prim_of(fh, uh);
-                L.hph[slot][h] = fh.x;
+                hfr[k] = plane_load(fp, hoff[k]);
+                hur[k] = plane_load(up, hoff[k]);
             }
         }
+    };
+    auto halo_store = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < HPT; ++k) {
+            if (hlds[k] >= 0) {
+                const int h = tid + k * NT;
+                L.hq[slot][h] = prim_of(hfr[k], hur[k]);
+                L.hph[slot][h] = hfr[k].x;
+            }
+        }
+    };
+    auto halo_ahead = [&](int zz, int slot) {
+        halo_load(zz);
+        halo_store(slot);
     };
     halo_ahead(zb, 0);
     if (zb + 1 < ze) halo_ahead(wrapi(zb + 1, P.nz), 1);
@@ -500,7 +513,11 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         ur[2] = plane_load(fluid_at(fluid_in, zp2), col);
         const V4<R> q2 = prim_of(fr[2], ur[2]);
         qr[2] = q2;
-        if (z + 2 < ze) halo_ahead(zp2, hs == 0 ? 2 : hs - 1);
+        // halo of plane z+2: loads issued here, recovered after the x and y
+        // faces, so the loads are in flight across the LDS fill, the barrier
+        // and the faces (a barrier waits for LDS, not for memory)
+        const bool halo_next = z + 2 < ze;
+        if (halo_next) halo_load(zp2);
 
         // (b) plane z into LDS: interior from registers, halo from the ring
         L.q[ty + 2][tx + 2] = qz;
@@ -554,6 +571,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
                 L.fx[c][xe] = ft;
             }
         }
+        if (halo_next) halo_store(hs == 0 ? 2 : hs - 1);
         // own +z face
         const V4<R> hz1 = half_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
         const V4<R> fzp = face_flux_h<2, M, R, E>(qz, hz, qz1, hz1, f0.x, f1.x, P);
