@@ -697,6 +697,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int b = blockIdx.x;
     if ((ntasks & 7) == 0) b = (b & 7) * (ntasks >> 3) + (b >> 3);
     const Task t = tasks[b];
+    if (t.zb >= t.ze) return;  // padding task (the list is padded to a multiple of 8)
     const BoxEvents &ev = events[t.box];
     if (!ev.active || ev.steps_done >= ev.steps_target) return;  // uniform per block
     const Box<R> &bx = boxes[t.box];

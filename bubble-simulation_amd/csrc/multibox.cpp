@@ -107,6 +107,14 @@ int Engine::upload() {
                     tasks.push_back(t);
                 }
     }
+    // pad to a multiple of 8 with empty tasks, so that the kernel can hand
+    // each XCD a contiguous run of tiles (blocks go round-robin over the 8
+    // XCDs; neighbouring tiles then share that XCD's L2)
+    while (!tasks.empty() && tasks.size() % 8 != 0) {
+        Task t;
+        std::memset(&t, 0, sizeof t);
+        tasks.push_back(t);
+    }
     ntasks = (int)tasks.size();
     if (n > cap_boxes) {
         if (d_boxes) (void)hipFree(d_boxes);
