@@ -163,7 +163,7 @@ struct BagEos {
         return R(1) / R(3);
     }
     // eos_enthalpy_from_pressure (eos.wgsl:56-59): w = 4p.
-    template <class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &) {
+    template <class M, class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &) {
         return R(4) * p;
     }
     // eos_energy_from_pressure (eos.wgsl:61-63): e = 3p.
@@ -197,8 +197,11 @@ struct GammaLawEos {
     template <class R> __host__ __device__ static inline R sound_speed2(R /*p*/, R /*phi*/, const Params<R> &P) {
         return P.eos_gamma - R(1);
     }
-    template <class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &P) {
-        return P.eos_gamma * p / (P.eos_gamma - R(1));
+    // FastMath: p times Gamma / (Gamma - 1), a factor formed once per
+    // launch (loop-invariant), instead of a division per call
+    template <class M, class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &P) {
+        if constexpr (M::fast) return p * M::div(P.eos_gamma, P.eos_gamma - R(1));
+        else return P.eos_gamma * p / (P.eos_gamma - R(1));
     }
     template <class R> __host__ __device__ static inline R energy(R p, R /*phi*/, const Params<R> &P) {
         return p / (P.eos_gamma - R(1));
@@ -271,7 +274,7 @@ __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params
 template <class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> eos_cons_from_prim(const V4<R> &q, R phi, const Params<R> &P) {
     const R v2 = M::min(dot3(q.x, q.y, q.z, q.x, q.y, q.z), P.z_max * P.z_max);
-    const R w_gam2 = M::div(E::enthalpy(q.w, phi, P), R(1) - v2);
+    const R w_gam2 = M::div(E::template enthalpy<M>(q.w, phi, P), R(1) - v2);
     return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 
@@ -348,7 +351,7 @@ template <class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> eos_cons_from_prim_v2(const V4<R> &q, R v2raw, R phi, const Params<R> &P,
                                                        R &w_gam2) {
     const R v2 = capped_v2<M>(v2raw, P);
-    w_gam2 = M::div(E::enthalpy(q.w, phi, P), R(1) - v2);
+    w_gam2 = M::div(E::template enthalpy<M>(q.w, phi, P), R(1) - v2);
     return V4<R>{w_gam2 - q.w, w_gam2 * q.x, w_gam2 * q.y, w_gam2 * q.z};
 }
 
@@ -403,8 +406,8 @@ template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
     if constexpr (M::fast) {
-        const R wl = M::div(E::enthalpy(ql.w, phil, P), R(1) - capped_v2<M>(v2l, P));
-        const R wr = M::div(E::enthalpy(qr.w, phir, P), R(1) - capped_v2<M>(v2r, P));
+        const R wl = M::div(E::template enthalpy<M>(ql.w, phil, P), R(1) - capped_v2<M>(v2l, P));
+        const R wr = M::div(E::template enthalpy<M>(qr.w, phir, P), R(1) - capped_v2<M>(v2r, P));
         R llm, llp, lrm, lrp;
         eos_wave_speeds_v2<A, M, R, E>(ql, v2l, phil, P, llm, llp);
         eos_wave_speeds_v2<A, M, R, E>(qr, v2r, phir, P, lrm, lrp);
