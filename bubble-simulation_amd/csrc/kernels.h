@@ -245,14 +245,15 @@ __device__ __forceinline__ global_t<char> *uniform_ptr(const void *p) {
     return reinterpret_cast<global_t<char> *>(((uint64_t)hi << 32) | lo);
 }
 
-// Element `idx` of a plane that starts at `base` (the same in every lane):
-// the byte offset is formed in 32 bits (a plane is well below 4 GiB), so the
-// access uses the scalar base + vector offset addressing mode.  Accesses go
-// through the clang vector types, which load and store as one dwordx2 /
-// dwordx4; the streaming forms are non-temporal, for data used once per
-// stage -- U^n and the output slot -- so they do not evict from L2 the U_in
-// planes that the halo reads and the update's re-read come back to
-// (measured +3 % at 512^3).
+// Element `idx` of a plane (the plane's base is the same in every lane): the
+// byte offset is formed in 32 bits (spec validation keeps an x-y plane below
+// 4 GiB).  Loads and stores move one dwordx2 / dwordx4 per element; the
+// streaming forms are non-temporal, for data used once per stage -- U^n and
+// the output slot -- so they do not evict from L2 the U_in planes that the
+// halo reads come back to (measured +3 % at 512^3).  Two implementations:
+// buffer resources (default) and, with -DBS_GLOBAL_PLANE_ACCESS, global
+// pointers with a scalar base (8 64-bit vector adds per plane; 0.5 % slower
+// at 512^3, 7 % with the gamma-law EOS).
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
