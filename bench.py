@@ -198,6 +198,42 @@ def copy_calibration(nbytes=1 << 30, repeats=3):
     return 0
 
 
+def copy_bandwidth(nbytes=1 << 30, repeats=5):
+    """Achieved HBM bandwidth of the runtime's own device-to-device copy
+    (read + write bytes per second), timed with HIP events: the practical
+    ceiling the stage kernel's traffic is compared against."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    size = ctypes.c_size_t(nbytes)
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        if hip.hipMalloc(ctypes.byref(a), size) or hip.hipMalloc(ctypes.byref(b), size):
+            return None
+        hip.hipMemset(a, 1, size)
+        hip.hipMemcpy(b, a, size, 3)  # warm
+        hip.hipEventCreate(ctypes.byref(ev0))
+        hip.hipEventCreate(ctypes.byref(ev1))
+        hip.hipEventRecord(ev0, None)
+        for _ in range(repeats):
+            hip.hipMemcpyAsync(b, a, size, 3, None)  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(ev1, None)
+        hip.hipEventSynchronize(ev1)
+        ms = ctypes.c_float()
+        if hip.hipEventElapsedTime(ctypes.byref(ms), ev0, ev1) or ms.value <= 0:
+            return None
+        return 2.0 * nbytes * repeats / (ms.value / 1e3) / 1e9
+    finally:
+        if ev0:
+            hip.hipEventDestroy(ev0)
+        if ev1:
+            hip.hipEventDestroy(ev1)
+        if a:
+            hip.hipFree(a)
+        if b:
+            hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +290,10 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
     sim.close()
+    copy_gbs = copy_bandwidth()
+    if copy_gbs:
+        log(f"bench: device copy {copy_gbs:.0f} GB/s (read + write); stage kernel traffic "
+            + (f"{traffic / avg_launch_s / 1e9:.0f} GB/s" if traffic else "not measured for this workload"))
     batch = None if args.no_batch else batched_sweep(bs)
     served = None if args.no_serve else served_stream(bs)
     cpu = None if args.no_cpu else cpu_baseline(bs)
@@ -278,6 +318,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
+        "hbm_context": {"device_copy_GBps": copy_gbs,
+                        "stage_traffic_GBps": traffic / avg_launch_s / 1e9 if traffic else None,
+                        "note": "1 GiB hipMemcpy device-to-device, read + write bytes; stage traffic = "
+                                "roofline.traffic / stage_kernel_ms_per_launch"},
         "batched_sweep": batch,
         "served": served,
     }

@@ -408,7 +408,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // the component-swapped states (vx <-> vy), so that round is one uniform
     // code path instead of one per axis.  The swap commutes with every
     // operation of the face (the sums of squares add the same two products
-    // in either order), so the result is the y face's, bit for bit.
+    // in either order), so under ExactMath the result is the y face's, bit
+    // for bit; under FastMath, FMA contraction rounds vx^2 + vy^2 in the
+    // other order, so those faces agree to rounding.
     // FastMath shares each cell's x half slope with the face on its left:
     // the cell forms q - h (the right state of its left face) and q + h (the
     // left state of its right face), and the lane to its right takes the
