@@ -234,6 +234,28 @@ def copy_bandwidth(nbytes=1 << 30, repeats=5):
             hip.hipFree(b)
 
 
+def hbm_context(bs, traffic, read_share, avg_launch_s):
+    """The practical HBM ceilings beside the published peak: streaming read,
+    write and copy kernels (bs_hbm_probe), the runtime's device copy, and the
+    rate those imply for the stage kernel's own mix of reads and writes,
+    against which its measured traffic rate is set."""
+    probe = bs.hbm_probe()
+    out = dict(probe)
+    out["device_copy_GBps"] = copy_bandwidth()
+    out["stage_traffic_GBps"] = traffic / avg_launch_s / 1e9 if traffic else None
+    out["stage_read_share"] = read_share
+    mix = None
+    if read_share is not None and probe["read_GBps"] > 0 and probe["write_GBps"] > 0:
+        mix = 1.0 / (read_share / probe["read_GBps"] + (1.0 - read_share) / probe["write_GBps"])
+    out["mix_ceiling_GBps"] = mix
+    out["stage_frac_of_mix_ceiling"] = out["stage_traffic_GBps"] / mix if mix and traffic else None
+    log(f"bench: HBM streaming read {probe['read_GBps']:.0f}, write {probe['write_GBps']:.0f}, "
+        f"copy {probe['copy_GBps']:.0f} GB/s; device copy {out['device_copy_GBps'] or 0:.0f} GB/s"
+        + (f"; stage traffic {out['stage_traffic_GBps']:.0f} GB/s = {out['stage_frac_of_mix_ceiling']:.2f} of "
+           f"the {mix:.0f} GB/s its read/write mix allows" if mix and traffic else ""))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,15 +307,18 @@ def main():
     bytes_per_launch = BYTES_PER_SITE_UPDATE / 3 * cells
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
+    read_share = None
     tfile = os.path.join(ROOT, "profiles", "traffic_fused_stage_512.json")
     if os.path.exists(tfile) and n == 512 and not eos:  # measured on the headline workload only
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            t = json.load(f)
+        traffic = t.get("bytes_per_launch")
+        kib = 1024.0
+        rd = (t["stage1"]["fetch_counter_kib"] + 2 * t["stage23"]["fetch_counter_kib"]) / 3 * kib * t["fetch_scale"]
+        wr = (t["stage1"]["write_counter_kib"] + 2 * t["stage23"]["write_counter_kib"]) / 3 * kib * t["write_scale"]
+        read_share = rd / (rd + wr)
     sim.close()
-    copy_gbs = copy_bandwidth()
-    if copy_gbs:
-        log(f"bench: device copy {copy_gbs:.0f} GB/s (read + write); stage kernel traffic "
-            + (f"{traffic / avg_launch_s / 1e9:.0f} GB/s" if traffic else "not measured for this workload"))
+    hbm = hbm_context(bs, traffic, read_share, avg_launch_s)
     batch = None if args.no_batch else batched_sweep(bs)
     served = None if args.no_serve else served_stream(bs)
     cpu = None if args.no_cpu else cpu_baseline(bs)
@@ -318,10 +343,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
-        "hbm_context": {"device_copy_GBps": copy_gbs,
-                        "stage_traffic_GBps": traffic / avg_launch_s / 1e9 if traffic else None,
-                        "note": "1 GiB hipMemcpy device-to-device, read + write bytes; stage traffic = "
-                                "roofline.traffic / stage_kernel_ms_per_launch"},
+        "hbm_context": hbm,
         "batched_sweep": batch,
         "served": served,
     }

@@ -98,6 +98,10 @@ class CSpec(C.Structure):
                 ("eos_gamma", C.c_float), ("dof_ratio", C.c_float)]
 
 
+class CHbmRates(C.Structure):
+    _fields_ = [("read_gbps", C.c_double), ("write_gbps", C.c_double), ("copy_gbps", C.c_double)]
+
+
 class CServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("reserve_bytes", C.c_uint64), ("max_active", C.c_uint32),
                 ("chunk_steps", C.c_uint32), ("precision", C.c_int32), ("math", C.c_int32)]
@@ -149,6 +153,7 @@ _SIGS = {
     "bs_device_count": (C.c_int, []),
     "bs_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bs_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "bs_hbm_probe": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(CHbmRates)]),
     "bs_sim_create": (C.c_int, [C.POINTER(CSpec), C.POINTER(_P)]),
     "bs_sim_destroy": (None, [_P]),
     "bs_sim_reset": (C.c_int, [_P, C.POINTER(CEvent), C.c_uint32]),
@@ -410,6 +415,14 @@ def device_name() -> str:
     buf = C.create_string_buffer(256)
     _check(lib().bs_device_name(buf, 256))
     return buf.value.decode()
+
+
+def hbm_probe(nbytes: int = 2 << 30, repeats: int = 5) -> dict:
+    """Achievable HBM read / write / copy bandwidth of the current device,
+    GB/s (bs_hbm_probe)."""
+    r = CHbmRates()
+    _check(lib().bs_hbm_probe(nbytes, repeats, C.byref(r)))
+    return {"read_GBps": r.read_gbps, "write_GBps": r.write_gbps, "copy_GBps": r.copy_gbps}
 
 
 def diag_dict(d: CDiagnostics) -> dict:

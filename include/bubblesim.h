@@ -299,6 +299,20 @@ int bs_device_memory(uint64_t *free_out, uint64_t *total_out);
 /* Device name into buf. */
 int bs_device_name(char *buf, size_t len);
 
+/* Achievable HBM bandwidth of the current device, in GB/s (1e9 B/s):
+ * streaming read-only, write-only and copy kernels over two buffers of
+ * `bytes` each (>= 1 MiB; far above the caches, e.g. 2 GiB), `repeats`
+ * launches per measurement, best over grids of 8..64 workgroups per CU.
+ * No counterpart in the reference: it gives the practical ceiling that the
+ * stage kernel's measured HBM traffic is compared against, beside the
+ * published peak.  Allocates and frees its buffers. */
+typedef struct bs_hbm_rates {
+    double read_gbps;
+    double write_gbps;
+    double copy_gbps; /* bytes read + bytes written, per second */
+} bs_hbm_rates;
+int bs_hbm_probe(uint64_t bytes, uint32_t repeats, bs_hbm_rates *out);
+
 /* ------------------------------------------------------------------------ */
 /*  Simulation (Simulation, sim.rs:180-1026)                                */
 /* ------------------------------------------------------------------------ */

@@ -123,3 +123,12 @@ def test_no_device_fails_loudly_without_gpu(bs):
     with pytest.raises(bs.BubbleSimError) as e:
         bs.Simulation(bs.make_spec(grid=16))
     assert e.value.code == bs.BS_ERR_DEVICE
+
+
+def test_hbm_probe_rejects_bad_arguments_before_touching_the_device(bs):
+    # argument checks run on the host, so this holds without a GPU
+    r = bs.CHbmRates()
+    L = bs.lib()
+    assert L.bs_hbm_probe(1024, 3, C.byref(r)) == bs.BS_ERR_INVALID
+    assert L.bs_hbm_probe(1 << 30, 0, C.byref(r)) == bs.BS_ERR_INVALID
+    assert L.bs_hbm_probe(1 << 30, 3, None) == bs.BS_ERR_INVALID
