@@ -190,6 +190,7 @@ _SIGS = {
     "bs_batch_total_cells": (C.c_uint64, [_P]),
     "bs_batch_diagnostics": (C.c_int, [_P, C.c_uint32, C.POINTER(CDiagnostics)]),
     "bs_batch_download": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "bs_batch_upload": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "bs_batch_steps": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "bs_batch_tile": (C.c_int, [_P, _u32p, _u32p]),
     "bs_server_create": (C.c_int, [C.POINTER(CServerConfig), C.POINTER(_P)]),
@@ -645,6 +646,15 @@ class Batch:
         _check(lib().bs_batch_download(self._h, box, field.ctypes.data_as(C.c_void_p),
                                        fluid.ctypes.data_as(C.c_void_p)))
         return field, fluid
+
+    def upload(self, box: int, field: np.ndarray, fluid: np.ndarray):
+        s = self.specs[box]
+        n = int(s.grid[0]) * int(s.grid[1]) * int(s.grid[2])
+        field = np.ascontiguousarray(field, dtype=self.real)
+        fluid = np.ascontiguousarray(fluid, dtype=self.real)
+        assert field.size == 2 * n and fluid.size == 4 * n
+        _check(lib().bs_batch_upload(self._h, box, field.ctypes.data_as(C.c_void_p),
+                                     fluid.ctypes.data_as(C.c_void_p)))
 
 
 class Server:

@@ -127,6 +127,19 @@ int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid) {
     return download_state(b->entries[box]->bx, b->stream, field, fluid);
 }
 
+int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fluid) {
+    if (!b) return set_error(BS_ERR_INVALID, "null batch");
+    if (box >= b->entries.size()) return set_error(BS_ERR_NOT_FOUND, "no such box");
+    int rc = batch_sync_progress(b);  // the slot holding U^n of that box
+    if (rc) return rc;
+    BoxState &bx = b->entries[box]->bx;
+    const size_t r = real_size(bx.precision);
+    BS_HIP_TRY(hipStreamSynchronize(b->stream));
+    if (field) BS_HIP_TRY(hipMemcpy(bx.field[bx.cur], field, bx.n_cells * 2 * r, hipMemcpyHostToDevice));
+    if (fluid) BS_HIP_TRY(hipMemcpy(bx.fluid[bx.cur], fluid, bx.n_cells * 4 * r, hipMemcpyHostToDevice));
+    return BS_OK;
+}
+
 int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out) {
     if (!b || !steps_out) return set_error(BS_ERR_INVALID, "null argument");
     if (box >= b->entries.size()) return set_error(BS_ERR_NOT_FOUND, "no such box");

@@ -401,6 +401,10 @@ uint32_t bs_batch_size(const bs_batch *b);
 uint64_t bs_batch_total_cells(const bs_batch *b);
 int bs_batch_diagnostics(bs_batch *b, uint32_t box, bs_diagnostics *out);
 int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid);
+/* Load U^n of one box (layout as bs_sim_upload).  Boxes never share a task,
+ * so a box's state reaches no other box: this is how a host forces a failing
+ * (e.g. non-finite) box to check that its neighbours are untouched. */
+int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fluid);
 int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out);
 /* Tile shape chosen by the autotuner for this batch (x, y). */
 int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty);
