@@ -64,7 +64,7 @@ def test_sub_critical_seeds_are_allowed_so_collapse_can_be_shown(bs):
 
 
 def test_defaults_match_the_documented_values(bs):
-    # SURVEY/physics.rs:70-140 defaults: eps 0.0125, m2 0.175, delta 0.675,
+    # physics.rs:70-140 defaults: eps 0.0125, m2 0.175, delta 0.675,
     # l_w 4.0, sigma 0.0833, R_c 13.33, R_0 15.33, e_ref = a_rad 0.125, dt 0.2
     d = bs.Model().derived()
     exp = dict(eps=0.0125, m2=0.175, delta=0.675, wall_width=4.0, surface_tension=1 / 12, critical_radius=40 / 3,
