@@ -234,6 +234,16 @@ def copy_bandwidth(nbytes=1 << 30, repeats=5):
             hip.hipFree(b)
 
 
+def side_measurement(name, fn, *args):
+    """Run one of the bench's side measurements; on failure log it and
+    report {"error": ...} in its place."""
+    try:
+        return fn(*args)
+    except Exception as e:  # noqa: BLE001 -- any failure is reported, not raised
+        log(f"bench: {name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def hbm_context(bs, traffic, read_share, avg_launch_s):
     """The practical HBM ceilings beside the published peak: streaming read,
     write and copy kernels (bs_hbm_probe), the runtime's device copy, and the
@@ -318,10 +328,11 @@ def main():
         wr = (t["stage1"]["write_counter_kib"] + 2 * t["stage23"]["write_counter_kib"]) / 3 * kib * t["write_scale"]
         read_share = rd / (rd + wr)
     sim.close()
-    hbm = hbm_context(bs, traffic, read_share, avg_launch_s)
-    batch = None if args.no_batch else batched_sweep(bs)
-    served = None if args.no_serve else served_stream(bs)
-    cpu = None if args.no_cpu else cpu_baseline(bs)
+    # the side measurements never take the headline line down with them
+    hbm = side_measurement("HBM context", hbm_context, bs, traffic, read_share, avg_launch_s)
+    batch = None if args.no_batch else side_measurement("batched sweep", batched_sweep, bs)
+    served = None if args.no_serve else side_measurement("served stream", served_stream, bs)
+    cpu = None if args.no_cpu else side_measurement("CPU baseline", cpu_baseline, bs)
     out = {
         "metric": "RK3 site-updates/s at 512^3 f32 (fused stage kernel, % of HBM peak)",
         "value": value,
