@@ -82,7 +82,7 @@ static hipError_t fused_one(const Box<R> &box, int stage, int zc, hipStream_t st
     const int nzc = (box.P.nz + zc - 1) / zc;
     const int nblocks = ntx * nty * nzc;
     hipLaunchKernelGGL((stage == 0 ? k_stage_fused<TX, TY, M, R, true, E> : k_stage_fused<TX, TY, M, R, false, E>),
-                       dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc, nblocks);
+                       dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc);
     return hipGetLastError();
 }
 

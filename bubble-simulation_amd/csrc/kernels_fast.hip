@@ -14,7 +14,7 @@ static hipError_t fused_one(const Box<float> &box, int stage, int zc, hipStream_
     const int nblocks = ntx * nty * nzc;
     hipLaunchKernelGGL((stage == 0 ? k_stage_fused<TX, TY, FastMathF32, float, true, E>
                                    : k_stage_fused<TX, TY, FastMathF32, float, false, E>),
-                       dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc, nblocks);
+                       dim3(nblocks), dim3(TX * TY), 0, st, box, stage, ntx, nty, zc);
     return hipGetLastError();
 }
 
