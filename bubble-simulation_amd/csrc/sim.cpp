@@ -118,6 +118,8 @@ int record_nucleation(bs_sim *s) {
 
 // record_steps (sim.rs:829-841)
 int record_steps(bs_sim *s, uint32_t n) {
+    if (!s->bx.field[0] || (s->spec.kernel == BS_KERNEL_TWO_PASS && !s->prim))
+        return set_error(BS_ERR_INVALID, "the simulation holds no lattice (a resize failed to allocate)");
     s->nucleated_this_frame = false;
     sim_params_fill(s->model, s->spec.grid, s->time, &s->bx.params);
     int rc = record_nucleation(s);
@@ -305,7 +307,16 @@ int bs_sim_resize(bs_sim *s, const uint32_t grid[3]) {
     sim_free(s);  // release first: peak memory is max(old, new)
     s->spec = ns;
     rc = sim_alloc(s);
-    if (rc) return rc;
+    if (rc) {
+        // Keep the handle usable: fall back to the minimum 8^3 lattice with
+        // the same parameters, which is where the reference's resize stands
+        // when its final allocation fails (sim.rs:756-760).
+        const std::string msg = bs_last_error();
+        sim_free(s);
+        s->spec.grid[0] = s->spec.grid[1] = s->spec.grid[2] = 8;
+        if (spec_schedule(s->spec, s->schedule, err) == BS_OK && sim_alloc(s) == BS_OK) (void)sim_reset(s);
+        return set_error(rc, msg);
+    }
     s->schedule = sched;
     return sim_reset(s);
 }

@@ -346,8 +346,11 @@ int bs_sim_request_diagnostics(bs_sim *sim);
 int bs_sim_poll_diagnostics(bs_sim *sim, bs_diagnostics *out);
 /* Block until all queued work on the simulation's stream has finished. */
 int bs_sim_synchronize(bs_sim *sim);
-/* Simulation::resize (sim.rs:736-762): validates first; on failure the
- * simulation is unchanged.  Re-initialises the lattice. */
+/* Simulation::resize (sim.rs:736-762): validates first; a rejected size
+ * leaves the simulation unchanged.  Re-initialises the lattice.  If the new
+ * lattice then fails to allocate, the simulation is left at the minimum 8^3
+ * lattice with its other parameters, as the reference's is (sim.rs:756-760),
+ * and the error is returned. */
 int bs_sim_resize(bs_sim *sim, const uint32_t grid[3]);
 /* Simulation::nucleate_now (sim.rs:1012-1025): queue a bubble at `pos`
  * (lattice cells) stamped before the next step. */
