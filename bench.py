@@ -10,8 +10,10 @@ steps where bubbles appear) of the whole box; the lattice is resident in HBM.
 
 Prints ONE JSON line: site-updates/s (cells x steps / s), the roofline of the
 fused stage kernel (192 algorithmic bytes per site-update against 8 TB/s),
-and the CPU baseline (the oracle's f32 restatement of the reference scheme on
-the host cores, timed on a bounded 128^3 sample).
+the CPU baseline (the oracle's f32 restatement of the reference scheme on
+the host cores, timed on a bounded 128^3 sample), and the accuracy of the
+measured f32 path: its field error against the same box stepped in f64, and
+the energy drift of both.
 """
 import argparse
 import json
@@ -38,11 +40,83 @@ def sweep_eta(bs, seed=7):
     return math.exp(math.log(0.02) + u * (math.log(1.0) - math.log(0.02)))
 
 
-def headline_spec(bs, n=512, eos=0, eos_gamma=4.0 / 3.0):
+def headline_spec(bs, n=512, eos=0, eos_gamma=4.0 / 3.0, precision=None, math_mode=None):
     model = bs.Model(eta=sweep_eta(bs))
+    prec = bs.PRECISION_F32 if precision is None else precision
     return bs.make_spec(grid=n, model=model, nucleation=bs.NUC_UNIFORM_STEPS, bubbles=32, seed=1,
-                        uniform_max_step=200, time_seed=11, precision=bs.PRECISION_F32, math=bs.MATH_FAST,
+                        uniform_max_step=200, time_seed=11, precision=prec,
+                        math=bs.MATH_FAST if math_mode is None else math_mode,
                         kernel=bs.KERNEL_FUSED, eos=eos, eos_gamma=eos_gamma)
+
+
+ACC_STEPS = 260  # every bubble of the headline schedule has nucleated by step 200
+BLOCKS = (("field", ("phi", "pi")), ("fluid", ("E", "Zx", "Zy", "Zz")))
+
+
+def block_errors(a32, a64, names):
+    """Squared-difference and squared-reference sums, and the largest
+    |difference| and |reference|, of each component of an f32 state against
+    the f64 one, accumulated in float64 one z plane at a time."""
+    k = a64.shape[-1]
+    se, sr, me, mr = np.zeros(k), np.zeros(k), np.zeros(k), np.zeros(k)
+    for z in range(a64.shape[0]):
+        r = a64[z].reshape(-1, k)
+        d = a32[z].reshape(-1, k).astype(np.float64) - r
+        se += np.einsum("ij,ij->j", d, d)
+        sr += np.einsum("ij,ij->j", r, r)
+        me = np.maximum(me, np.abs(d).max(axis=0))
+        mr = np.maximum(mr, np.abs(r).max(axis=0))
+    comps = {nm: {"rel_l2": float(np.sqrt(se[i] / sr[i])) if sr[i] > 0 else None,
+                  "rel_linf": float(me[i] / mr[i]) if mr[i] > 0 else None} for i, nm in enumerate(names)}
+    block = {"rel_l2": float(np.sqrt(se.sum() / sr.sum())), "rel_linf": float(me.max() / mr.max())}
+    return block, comps
+
+
+def accuracy(bs, n, eos, eos_gamma, steps=ACC_STEPS, chunk=20):
+    """Field error and energy drift of the measured path.  The headline box
+    is stepped from the same schedule in f64 (the ExactMath fused kernel:
+    the oracle's scheme and operation order, which it equals bit for bit
+    wherever the oracle runs), in f32 FastMath (the measured path) and in f32
+    ExactMath (the reference's f32 operation order, for scale).  Reported:
+    relative L2 and L-infinity of each f32 state against f64 after `steps`
+    steps, per block (phi/pi and the fluid) and per component, and each run's
+    energy drift E/E0 - 1, E0 taken at the first report after the last
+    nucleation (headless.rs:54-78)."""
+    runs = (("f32_fast", bs.PRECISION_F32, bs.MATH_FAST), ("f32_exact", bs.PRECISION_F32, bs.MATH_EXACT),
+            ("f64", bs.PRECISION_F64, bs.MATH_EXACT))
+    states, drift = {}, {}
+    base_step = None
+    for name, prec, mm in runs:
+        sim = bs.Simulation(headline_spec(bs, n, eos, eos_gamma, precision=prec, math_mode=mm))
+        try:
+            base, done = None, 0
+            while done < steps:
+                c = min(chunk, steps - done)
+                sim.advance(c)
+                done += c
+                if base is None and sim.remaining_bubbles() == 0:
+                    base, base_step = sim.diagnostics().mean_energy, done
+            d = sim.diagnostics()
+            if d.nonfinite:
+                raise RuntimeError(f"{name} run went non-finite")
+            drift[name] = d.mean_energy / base - 1.0 if base else None
+            states[name] = sim.download()
+        finally:
+            sim.close()
+    out = {"reference": "f64 ExactMath fused kernel (bit-identical to the oracle at oracle sizes)", "steps": steps,
+           "drift_from_step": base_step, "energy_drift": drift, "blocks": {}, "components": {}}
+    for name in ("f32_fast", "f32_exact"):
+        out["blocks"][name], out["components"][name] = {}, {}
+        for k, (bname, names) in enumerate(BLOCKS):
+            blk, comps = block_errors(states[name][k], states["f64"][k], names)
+            out["blocks"][name][bname] = blk
+            out["components"][name].update(comps)
+    fb, eb = out["blocks"]["f32_fast"], out["blocks"]["f32_exact"]
+    log(f"bench: accuracy after {steps} steps against f64, rel L2 field / fluid: f32 fast "
+        f"{fb['field']['rel_l2']:.2e} / {fb['fluid']['rel_l2']:.2e}, f32 exact {eb['field']['rel_l2']:.2e} / "
+        f"{eb['fluid']['rel_l2']:.2e}; energy drift from step {base_step}: "
+        + ", ".join(f"{k} {v:+.3e}" for k, v in drift.items() if v is not None))
+    return out
 
 
 def cpu_baseline(bs, budget_s=12.0):
@@ -284,6 +358,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the f32-against-f64 field error and drift")
     ap.add_argument("--eos", choices=["bag", "gamma-law"], default="bag",
                     help="equation of state of the stepped box (the headline is the bag model)")
     ap.add_argument("--eos-gamma", type=float, default=1.45, help="gamma-law adiabatic index")
@@ -333,6 +408,8 @@ def main():
     sim.close()
     # the side measurements never take the headline line down with them
     hbm = side_measurement("HBM context", hbm_context, bs, achieved, traffic, avg_launch_s)
+    acc = None if args.no_accuracy else side_measurement(
+        "accuracy", accuracy, bs, n, eos, args.eos_gamma if eos else 4.0 / 3.0)
     batch = None if args.no_batch else side_measurement("batched sweep", batched_sweep, bs)
     served = None if args.no_serve else side_measurement("served stream", served_stream, bs)
     cpu = None if args.no_cpu else side_measurement("CPU baseline", cpu_baseline, bs)
@@ -356,6 +433,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "accuracy": acc,
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
         "hbm_context": hbm,
         "batched_sweep": batch,
