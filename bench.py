@@ -119,28 +119,39 @@ def accuracy(bs, n, eos, eos_gamma, steps=ACC_STEPS, chunk=20):
     return out
 
 
-def cpu_baseline(bs, budget_s=12.0):
-    """The oracle (f32, reference operation order) on the host cores."""
+def cpu_baseline(bs, budget_s=12.0, single_budget_s=5.0):
+    """The oracle (f32, reference operation order) on the host cores: OpenMP
+    over this process's CPU share (OMP_NUM_THREADS, else the visible cores up
+    to 16), and on one thread beside it."""
     from oracle import cpu as orc
     n = 128
-    spec = headline_spec(bs, n)
     model = bs.Model(eta=sweep_eta(bs))
     params = bs.sim_params_words(model, (n, n, n))
     sched = bs.make_schedule(bs.NUC_UNIFORM_STEPS, 32, model.seed_radius(), n, 60.0, 1, 200, 11, model.dt())
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    o = orc.OracleSim(0, params, sched)
-    o.advance(1)  # warm
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        o.advance(1)
-        steps += 1
-    dt = time.perf_counter() - t0
-    o.close()
-    del spec
-    return {"value": n ** 3 * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n}^3 box of the same physics, {steps} RK3 steps in {dt:.1f} s, OpenMP {threads} threads"}
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(visible, 16)
+
+    def timed(nthreads, budget):
+        orc.set_threads(nthreads)
+        o = orc.OracleSim(0, params, sched)
+        o.advance(1)  # warm
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            o.advance(1)
+            steps += 1
+        dt = time.perf_counter() - t0
+        o.close()
+        return n ** 3 * steps / dt, steps, dt
+
+    rate, steps, dt = timed(threads, budget_s)
+    rate1, steps1, dt1 = timed(1, single_budget_s)
+    log(f"bench: CPU baseline {rate / 1e6:.2f} M/s on {threads} threads ({visible} cores visible), "
+        f"{rate1 / 1e6:.2f} M/s on 1")
+    return {"value": rate, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n}^3 box of the same physics, {steps} RK3 steps in {dt:.1f} s, OpenMP {threads} threads",
+            "host_cores_visible": visible, "single_thread_value": rate1,
+            "single_thread_sample": f"{steps1} RK3 steps of the same box in {dt1:.1f} s"}
 
 
 def sweep_specs(bs, n_boxes=64, master_seed=2024):

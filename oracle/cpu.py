@@ -17,6 +17,11 @@ LIB = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
+def set_threads(n: int) -> int:
+    """OpenMP worker threads of the oracle's loops; returns the count in force."""
+    return int(lib().oracle_set_threads(n))
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -29,6 +34,8 @@ def lib():
         L = C.CDLL(LIB)
         f32p = C.POINTER(C.c_float)
         f64p = C.POINTER(C.c_double)
+        L.oracle_set_threads.restype = C.c_int
+        L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_create.restype = C.c_void_p
         L.oracle_create.argtypes = [C.c_int, f32p, f32p, C.c_int]
         L.oracle_destroy.argtypes = [C.c_void_p]

@@ -23,6 +23,8 @@
 // and sqrt.  The WGSL `tanh` has implementation-defined accuracy; this file
 // uses expm1-based tanh (Cody-Waite reduction of 2|x| by ln 2, Taylor series
 // of expm1 on the reduced argument), which the tests pin against libm's tanh.
+#include <omp.h>
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -563,6 +565,13 @@ void diag_naive_f32(Lattice<float> &L, float *out4, uint32_t *vmax_bits) {
 //  ctypes surface
 // ---------------------------------------------------------------------------
 extern "C" {
+
+// Worker threads of the OpenMP loops that follow (n >= 1); returns the count
+// now in force.
+int oracle_set_threads(int n) {
+    if (n >= 1) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 // params: 24 words of the SimParams block; events: 5 floats each
 // (time, x, y, z, r0), already sorted by time.

@@ -174,3 +174,37 @@ def test_smallest_and_odd_boxes_match_oracle(bs, orc, gpu_available, grid):
         gf, gu = sim.download()
         of, ou = o.state()
         assert np.array_equal(gf, of) and np.array_equal(gu, ou)
+
+
+def test_centre_bubble_64_against_f64_over_300_steps(bs, gpu_available):
+    # 64^3, eta 0.3, eps_ratio 0.3, seed factor 1.15, one bubble at the
+    # centre cell (the --bubble-at-centre extension), 300 steps: the f32
+    # states (FastMath and the reference's f32 operation order) against the
+    # f64 solver at steps 100, 200 and 300, per state block, with the energy
+    # drift of each run from the first report after nucleation.
+    model = bs.Model(eta=0.3, eps_ratio=0.3, seed_factor=1.15)
+    runs = {}
+    for name, prec, math in (("fast", 0, bs.MATH_FAST), ("exact", 0, bs.MATH_EXACT), ("f64", 1, bs.MATH_EXACT)):
+        spec = bs.make_spec(grid=64, model=model, precision=prec, math=math, nucleation=bs.NUC_SIMULTANEOUS,
+                            bubbles=1, bubble_at_centre=True)
+        sim = bs.Simulation(spec)
+        states, energy = [], []
+        for _ in range(3):
+            sim.advance(100)
+            states.append(sim.download())
+            energy.append(sim.diagnostics().mean_energy)
+        sim.close()
+        runs[name] = (states, energy)
+    for r in range(3):
+        for k, block in enumerate(("phi/pi", "fluid")):
+            ref = runs["f64"][0][r][k]
+            e_fast = _rel_l2(runs["fast"][0][r][k], ref)
+            e_exact = _rel_l2(runs["exact"][0][r][k], ref)
+            _log(f"step {100 * (r + 1)} {block}: fast-vs-f64 {e_fast:.3e}  exact-f32-vs-f64 {e_exact:.3e}")
+            assert e_fast < 1e-5
+            assert e_fast <= 2.0 * e_exact + 1e-7
+    drift = {n: runs[n][1][2] / runs[n][1][0] - 1.0 for n in runs}
+    _log("energy drift, steps 100 to 300: " + ", ".join(f"{n} {d:+.3e}" for n, d in drift.items()))
+    # f32 drift is f64's plus f32 rounding; FastMath's no larger than twice
+    # the reference's f32 order's
+    assert abs(drift["fast"] - drift["f64"]) <= 2.0 * abs(drift["exact"] - drift["f64"]) + 1e-6
