@@ -23,7 +23,9 @@
 // and sqrt.  The WGSL `tanh` has implementation-defined accuracy; this file
 // uses expm1-based tanh (Cody-Waite reduction of 2|x| by ln 2, Taylor series
 // of expm1 on the reduced argument), which the tests pin against libm's tanh.
+#ifdef _OPENMP
 #include <omp.h>
+#endif
 
 #include <cmath>
 #include <cstdint>
@@ -569,8 +571,13 @@ extern "C" {
 // Worker threads of the OpenMP loops that follow (n >= 1); returns the count
 // now in force.
 int oracle_set_threads(int n) {
+#ifdef _OPENMP
     if (n >= 1) omp_set_num_threads(n);
     return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;  // built without OpenMP: the loops run on the calling thread
+#endif
 }
 
 // params: 24 words of the SimParams block; events: 5 floats each
