@@ -176,13 +176,14 @@ def sweep_specs(bs, n_boxes=64, master_seed=2024):
     return specs
 
 
-def batched_sweep(bs, steps=1000, seq_steps=40):
+def batched_sweep(bs, steps=1000, warm=25):
     """64 uneven boxes in one launch per RK stage (hipGraph replay) against
-    the same boxes solved one after another on the plain path."""
+    the same boxes solved one after another on the plain path, both over the
+    same steps (warm .. warm + steps) of every box."""
     specs = sweep_specs(bs)
     b = bs.Batch(specs)
     cells = b.total_cells()
-    b.advance(25, use_graph=True)
+    b.advance(warm, use_graph=True)
     b.synchronize()
     t0 = time.perf_counter()
     b.advance(steps, use_graph=True)
@@ -196,13 +197,13 @@ def batched_sweep(bs, steps=1000, seq_steps=40):
     seq_t, seq_work = 0.0, 0
     for s in specs:
         sim = bs.Simulation(s)
-        sim.advance(5)
+        sim.advance(warm)
         sim.synchronize()
         t0 = time.perf_counter()
-        sim.advance(seq_steps)
+        sim.advance(steps)
         sim.synchronize()
         seq_t += time.perf_counter() - t0
-        seq_work += sim.cell_count() * seq_steps
+        seq_work += sim.cell_count() * steps
         sim.close()
     sequential = seq_work / seq_t
     log(f"bench: sequential plain solves {sequential / 1e9:.2f} G/s")
@@ -260,18 +261,17 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     served = work / wall
     bwall, blat, _, bok = serve(bs, jobs, burst=True)
     burst = work / bwall
-    # sequential plain solves of a sample of the same jobs, rate extrapolated
+    # sequential plain solves of the first 8 of the same jobs, whole jobs
+    # from step 0 as the server runs them, rate extrapolated to the rest
     seq_t, seq_work = 0.0, 0
     for (_, spec, steps) in jobs[:8]:
         sim = bs.Simulation(spec)
-        n = min(steps, 200)
-        sim.advance(5)
         sim.synchronize()
         t1 = time.perf_counter()
-        sim.advance(n)
+        sim.advance(steps)
         sim.synchronize()
         seq_t += time.perf_counter() - t1
-        seq_work += sim.cell_count() * n
+        seq_work += sim.cell_count() * steps
         sim.close()
     sequential = seq_work / seq_t
     log(f"bench: served {n_jobs} jobs at {rate_hz} Hz: wall {wall:.1f} s, {served / 1e9:.2f} G/s, "
