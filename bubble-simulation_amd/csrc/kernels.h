@@ -431,15 +431,13 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     // ---- prologue: column state for zb-2 .. zb+1, the face zb-1/2 ----
     // (phi of plane z-1, U_in of planes z, z+1 and the primitives of z, z+1
     // stay in registers, so every input plane is read from memory once per
-    // column).  (Unrolling the march by three so that the ring shifts
-    // become register renames needs more than the 96 VGPRs of 5 waves per
-    // SIMD: it spills, or at 4 waves per SIMD runs 6 % slower.)
-    V2<R> fr[3], ftmp;  // field of planes z, z+1, z+2
+    // column).  Each ring has three slots, addressed by the plane's phase
+    // (z - zb) mod 3.
+    V2<R> fr[3], ftmp;  // field of planes z, z+1, z+2 (slot I2 holds z-1 until z+2 arrives)
     V4<R> ur[3], utmp;  // U_in of the same planes
     V4<R> qr[3];        // their primitives
-    V4<R> hr[2];        // z half slopes of planes z, z+1
-    V4<R> fzr[2];       // z faces z-1/2 and z+1/2
-    R ph_m;             // phi of plane z-1
+    V4<R> hr[3];        // z half slopes of planes z, z+1
+    V4<R> fzr[3];       // z faces z-1/2 and z+1/2
     auto field_at = [&](const V2<R> *a, int zz) { return plane_ref(a + (int64_t)zz * plane, (unsigned)plane); };
     auto fluid_at = [&](const V4<R> *a, int zz) { return plane_ref(a + (int64_t)zz * plane, (unsigned)plane); };
     ftmp = plane_load(field_at(field_in, wrapi(zb - 2, P.nz)), col);
@@ -448,7 +446,6 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     ftmp = plane_load(field_at(field_in, wrapi(zb - 1, P.nz)), col);
     utmp = plane_load(fluid_at(fluid_in, wrapi(zb - 1, P.nz)), col);
     const V4<R> qb = prim_of(ftmp, utmp);
-    ph_m = ftmp.x;
     fr[0] = plane_load(field_at(field_in, wrapi(zb, P.nz)), col);
     ur[0] = plane_load(fluid_at(fluid_in, wrapi(zb, P.nz)), col);
     qr[0] = prim_of(fr[0], ur[0]);
@@ -457,7 +454,8 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     qr[1] = prim_of(fr[1], ur[1]);
     const V4<R> h_b = half_slope4<M>(sub4(qb, qa), sub4(qr[0], qb));
     hr[0] = half_slope4<M>(sub4(qr[0], qb), sub4(qr[1], qr[0]));
-    fzr[0] = face_flux_h<2, M, R, E>(qb, h_b, qr[0], hr[0], ph_m, fr[0].x, P);
+    fzr[0] = face_flux_h<2, M, R, E>(qb, h_b, qr[0], hr[0], ftmp.x, fr[0].x, P);
+    fr[2] = ftmp;  // plane zb-1: its phi is the first plane's phi(z-1)
 
     V4<R> *qflat = &L.q[0][0];
     R *phiflat = &L.phi[0][0];
@@ -493,15 +491,21 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     halo_ahead(zb, 0);
     if (zb + 1 < ze) halo_ahead(wrapi(zb + 1, P.nz), 1);
 
-    // hs = (z - zb) mod 3 is the halo ring slot of plane z
-    for (int z = zb, hs = 0; z < ze; ++z) {
+    // One plane of the march.  The column rings hold planes z, z+1, z+2 in
+    // slots I0, I1, I2 = (z - zb) mod 3 and its successors, compile-time
+    // constants in the march below (unrolled by three), so that advancing
+    // the rings renames registers instead of moving them; I0 is also the
+    // halo ring slot of plane z.
+    auto march = [&](const int z, auto phase) {
+        constexpr int I0 = decltype(phase)::value, I1 = (I0 + 1) % 3, I2 = (I0 + 2) % 3;
         const int zp2 = z + 2 < P.nz ? z + 2 : z + 2 - P.nz;
-        const V2<R> f0 = fr[0], f1 = fr[1];
-        const V4<R> qz = qr[0], qz1 = qr[1], hz = hr[0], fzm = fzr[0];
+        const R ph_m = fr[I2].x;  // plane z-1, before plane z+2 takes its slot
+        const V2<R> f0 = fr[I0], f1 = fr[I1];
+        const V4<R> qz = qr[I0], qz1 = qr[I1], hz = hr[I0], fzm = fzr[I0];
         // the update's operand U^n(z) of this cell, loaded first so that it
         // arrives while the faces are computed (U_in(z) is in the ring)
         const R pi0 = f0.y;
-        const V4<R> uin = ur[0];
+        const V4<R> uin = ur[I0];
         V2<R> fnn;
         V4<R> un;
         if constexpr (S0) {
@@ -512,10 +516,10 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             un = stream_load(fluid_at(fluid_n, z), col);
         }
         // (a) plane z+2 of this column (for the +z face)
-        fr[2] = plane_load(field_at(field_in, zp2), col);
-        ur[2] = plane_load(fluid_at(fluid_in, zp2), col);
-        const V4<R> q2 = prim_of(fr[2], ur[2]);
-        qr[2] = q2;
+        fr[I2] = plane_load(field_at(field_in, zp2), col);
+        ur[I2] = plane_load(fluid_at(fluid_in, zp2), col);
+        const V4<R> q2 = prim_of(fr[I2], ur[I2]);
+        qr[I2] = q2;
         // halo of plane z+2: loads issued here, recovered after the x and y
         // faces, so the loads are in flight across the LDS fill, the barrier
         // and the faces (a barrier waits for LDS, not for memory)
@@ -529,8 +533,8 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
                 const int h = tid + k * NT;
-                qflat[hlds[k]] = L.hq[hs][h];
-                if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[hs][h];
+                qflat[hlds[k]] = L.hq[I0][h];
+                if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[I0][h];
             }
         }
         __syncthreads();
@@ -574,12 +578,12 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
                 L.fx[c][xe] = ft;
             }
         }
-        if (halo_next) halo_store(hs == 0 ? 2 : hs - 1);
+        if (halo_next) halo_store(I2);
         // own +z face
         const V4<R> hz1 = half_slope4<M>(sub4(qz1, qz), sub4(q2, qz1));
         const V4<R> fzp = face_flux_h<2, M, R, E>(qz, hz, qz1, hz1, f0.x, f1.x, P);
-        hr[1] = hz1;
-        fzr[1] = fzp;
+        hr[I1] = hz1;
+        fzr[I1] = fzp;
 
         // field stencil from LDS / registers (step.wgsl:118-122)
         const R pxm = L.phi[ty + 1][tx], pxp = L.phi[ty + 1][tx + 2];
@@ -627,27 +631,28 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             stream_store(fluid_at(fluid_out, z), col, fo);
         }
 
-
-        // (e) advance the column ring
-        ph_m = f0.x;
-        fr[0] = fr[1]; fr[1] = fr[2];
-        ur[0] = ur[1]; ur[1] = ur[2];
-        qr[0] = qr[1]; qr[1] = qr[2];
-        hr[0] = hr[1];
-        fzr[0] = fzr[1];
-        hs = hs == 2 ? 0 : hs + 1;
+    };
+    int z = zb;
+    for (; z + 3 <= ze; z += 3) {
+        march(z, std::integral_constant<int, 0>{});
+        march(z + 1, std::integral_constant<int, 1>{});
+        march(z + 2, std::integral_constant<int, 2>{});
     }
+    if (z < ze) march(z, std::integral_constant<int, 0>{});
+    if (z + 1 < ze) march(z + 1, std::integral_constant<int, 1>{});
 }
 
 // Minimum waves per SIMD the register allocator must leave room for: the
-// fast f32 stage fits 5 waves (<= 96 VGPRs) without spilling, and measures
-// faster there than at the 4 waves it would otherwise settle on; the exact
-// and f64 instantiations keep the compiler's choice.
+// fast f32 stage, with its march unrolled by three (so the column rings
+// advance by register renaming), needs 116-124 VGPRs, which 4 waves allow
+// without spilling (at 5 waves, 96 VGPRs, it spills); it runs about 0.5 %
+// faster than the rolled march at 5 waves.  The exact and f64
+// instantiations keep the compiler's choice.
 template <class M> struct Occupancy {
     static constexpr int waves = 1;
 };
 #ifndef BS_FAST_WAVES_PER_EU
-#define BS_FAST_WAVES_PER_EU 5
+#define BS_FAST_WAVES_PER_EU 4
 #endif
 template <> struct Occupancy<FastMathF32> {
     static constexpr int waves = BS_FAST_WAVES_PER_EU;
