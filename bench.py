@@ -5,14 +5,16 @@ Headline workload: a single 512^3 periodic f32 box, 32 bubbles with seeded
 centres (the reference's Pcg64Mcg stream, seed 1) and nucleation steps drawn
 uniformly in [0, 200], eta drawn log-uniformly in [0.02, 1] from a seeded
 sweep generator, default lambda / eps_ratio / alpha / cfl.  One "step" is one
-full RK3 step (three fused stage launches, plus the nucleation stamp on the
-steps where bubbles appear) of the whole box; the lattice is resident in HBM.
-The default window, steps 5 to 1005, covers the nucleation of every bubble
-and the transition to the broken phase: the stage kernel's instruction and
-byte counts do not depend on the state, but its clock does (the power limit
-allows a higher clock while the box is still nearly uniform), so a short
-window at the start would overstate the rate; the first 30 steps are
-reported separately for scale.
+full RK3 step (three fused stage launches) of the whole box; the lattice is
+resident in HBM.  Before the timed window, and outside timing, the box is
+advanced past its last nucleation (step 197 of this schedule), then by the
+warm-up steps: the stage kernel's instruction and byte counts do not depend
+on the state, but its clock does (the power limit allows a higher clock
+while the box is still nearly uniform), so a window at the start of the run
+would overstate the rate.  The default window (800 steps after the warm-up)
+runs through the rest of the transition to the broken phase; the first 30
+steps of the run and 30 steps after step 1000 (the box fully broken, the
+slowest state) are reported beside it for scale.
 
 Prints ONE JSON line: site-updates/s (cells x steps / s), the roofline of the
 fused stage kernel (192 algorithmic bytes per site-update against 8 TB/s),
@@ -44,6 +46,15 @@ def log(*a):
 def sweep_eta(bs, seed=7):
     u = float(bs.pcg64mcg_f32(seed, 1)[0])
     return math.exp(math.log(0.02) + u * (math.log(1.0) - math.log(0.02)))
+
+
+def settle_steps(bs, sim):
+    """Steps until every scheduled bubble of `sim` has been stamped: an
+    event due at step s is stamped before step s."""
+    sched = sim.schedule()
+    if len(sched) == 0:
+        return 0
+    return int(bs.event_steps(sched, float(sim.spec.model.cfl * sim.spec.model.dx)).max()) + 1
 
 
 def headline_spec(bs, n=512, eos=0, eos_gamma=4.0 / 3.0, precision=None, math_mode=None):
@@ -176,40 +187,102 @@ def sweep_specs(bs, n_boxes=64, master_seed=2024):
     return specs
 
 
-def batched_sweep(bs, steps=1000, warm=25):
+def field_errors(bs_states, ref_states):
+    """Per block (phi/pi, fluid): relative L2 and L-infinity of f32 states
+    against f64 ones of the same boxes, over all the boxes together."""
+    out = {}
+    for k, (bname, _) in enumerate(BLOCKS):
+        se = sr = 0.0
+        me = mr = 0.0
+        for a32, a64 in zip(bs_states, ref_states):
+            r = a64[k].astype(np.float64)
+            d = a32[k].astype(np.float64) - r
+            se += float(np.sum(d * d))
+            sr += float(np.sum(r * r))
+            me = max(me, float(np.abs(d).max()))
+            mr = max(mr, float(np.abs(r).max()))
+        out[bname] = {"rel_l2": float(np.sqrt(se / sr)) if sr > 0 else None, "rel_linf": me / mr if mr > 0 else None}
+    return out
+
+
+def f64_spec(bs, spec):
+    """The same box in f64 ExactMath."""
+    s64 = bs.CSpec.from_buffer_copy(spec)
+    s64.precision = bs.PRECISION_F64
+    s64.math = bs.MATH_EXACT
+    return s64
+
+
+def accuracy_sample(specs):
+    """Indices of the first box of each edge 32, 64, 96, 128 (or fewer)."""
+    idx, seen = [], set()
+    for i, s in enumerate(specs):
+        e = int(s.grid[0])
+        if e in (32, 64, 96, 128) and e not in seen:
+            seen.add(e)
+            idx.append(i)
+    return idx
+
+
+def batched_sweep(bs, steps=1000, report_at=200):
     """64 uneven boxes in one launch per RK stage (hipGraph replay) against
-    the same boxes solved one after another on the plain path, both over the
-    same steps (warm .. warm + steps) of every box."""
+    the same boxes solved one after another on the plain path, both as whole
+    jobs of `steps` steps from the initial state.  The batch reports
+    diagnostics of a sample of boxes (one per edge 32, 64, 96, 128) at step
+    `report_at` (every bubble has been stamped by then) and at the end, as
+    the headless run reports, inside its timed region; the same sample is
+    then run in f64 through the same engine for the field error and drift."""
     specs = sweep_specs(bs)
+    sample = accuracy_sample(specs)
     b = bs.Batch(specs)
     cells = b.total_cells()
-    b.advance(warm, use_graph=True)
     b.synchronize()
     t0 = time.perf_counter()
-    b.advance(steps, use_graph=True)
+    b.advance(report_at, use_graph=True)
+    e0 = {i: b.diagnostics(i).mean_energy for i in sample}
+    b.advance(steps - report_at, use_graph=True)
     b.synchronize()
     dt = time.perf_counter() - t0
-    tile = b.tile()
+    e1 = {i: b.diagnostics(i) for i in sample}
+    states = [b.download(i) for i in sample]
+    plan = b.plan()
+    graph_steps = b.graph_steps()
     b.close()
     batched = cells * steps / dt
     log(f"bench: batched {len(specs)} boxes, {cells / 1e6:.1f} M cells, {steps} steps in {dt:.2f} s "
-        f"= {batched / 1e9:.2f} G/s (tile {tile})")
+        f"= {batched / 1e9:.2f} G/s (plan {plan}, {graph_steps} steps by graph replay)")
+    if any(e1[i].nonfinite for i in sample):
+        raise RuntimeError("a batched box went non-finite")
+    # f64 of the sample through the same engine
+    b64 = bs.Batch([f64_spec(bs, specs[i]) for i in sample])
+    b64.advance(report_at, use_graph=True)
+    f0 = [b64.diagnostics(k).mean_energy for k in range(len(sample))]
+    b64.advance(steps - report_at, use_graph=True)
+    f1 = [b64.diagnostics(k).mean_energy for k in range(len(sample))]
+    ref = [b64.download(k) for k in range(len(sample))]
+    b64.close()
+    acc = {"reference": "the same boxes in f64 ExactMath through the batched engine", "steps": steps,
+           "boxes": [{"index": i, "edge": int(specs[i].grid[0]),
+                      "energy_drift_f32": e1[i].mean_energy / e0[i] - 1.0,
+                      "energy_drift_f64": f1[k] / f0[k] - 1.0} for k, i in enumerate(sample)],
+           "drift_from_step": report_at, "blocks": field_errors(states, ref)}
+    log("bench: batched accuracy (sample " + ", ".join(str(int(specs[i].grid[0])) for i in sample) + "): rel L2 "
+        + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()))
     seq_t, seq_work = 0.0, 0
     for s in specs:
         sim = bs.Simulation(s)
-        sim.advance(warm)
         sim.synchronize()
-        t0 = time.perf_counter()
+        t1 = time.perf_counter()
         sim.advance(steps)
         sim.synchronize()
-        seq_t += time.perf_counter() - t0
+        seq_t += time.perf_counter() - t1
         seq_work += sim.cell_count() * steps
         sim.close()
     sequential = seq_work / seq_t
     log(f"bench: sequential plain solves {sequential / 1e9:.2f} G/s")
     return {"boxes": len(specs), "cells": cells, "steps": steps, "site_updates_per_s": batched,
             "sequential_site_updates_per_s": sequential, "speedup_vs_sequential": batched / sequential,
-            "tile": list(tile)}
+            "plan": plan, "graph_replayed_steps": graph_steps, "accuracy": acc}
 
 
 def served_jobs(bs, n_jobs, rate_hz, seed):
@@ -231,40 +304,76 @@ def served_jobs(bs, n_jobs, rate_hz, seed):
     return jobs
 
 
-def serve(bs, jobs, burst):
+def serve(bs, jobs, burst, keep=()):
     """Submit the jobs to the continuous-batching server under a 288 GB -
     16 GB budget, at their arrival times or (burst) all at once; returns
-    (wall seconds, per-job latencies, server stats, all jobs ok)."""
+    (wall seconds, per-job results, server stats, kept states by job index)."""
     srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20)
-    t0 = time.perf_counter()
-    ids = []
-    for (ta, spec, steps) in jobs:
-        now = time.perf_counter() - t0
-        if not burst and ta > now:
-            time.sleep(ta - now)
-        ids.append(srv.submit(spec, steps))
-    res = [srv.wait(j, 600) for j in ids]
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    return wall, np.array([r.latency_s for r in res]), st, all(r.status == 0 for r in res)
+    try:
+        t0 = time.perf_counter()
+        ids = []
+        for k, (ta, spec, steps) in enumerate(jobs):
+            now = time.perf_counter() - t0
+            if not burst and ta > now:
+                time.sleep(ta - now)
+            ids.append(srv.submit(spec, steps, keep_state=k in keep))
+        res = [srv.wait(j, 600) for j in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        kept = {k: srv.fetch_state(ids[k], jobs[k][1]) for k in keep if res[k].status == 0}
+    finally:
+        srv.close()
+    return wall, res, st, kept
+
+
+def served_accuracy(bs, jobs, keep, kept, res):
+    """Field error and drift of sampled served jobs against the same jobs run
+    in f64 ExactMath through a server; the drift of each is taken from the
+    job's baseline diagnostics (first tick after its last nucleation) to its
+    final ones."""
+    srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20, precision=bs.PRECISION_F64,
+                    math=bs.MATH_EXACT)
+    try:
+        ids = [srv.submit(f64_spec(bs, jobs[k][1]), jobs[k][2], keep_state=True) for k in keep]
+        r64 = [srv.wait(j, 600) for j in ids]
+        ref = [srv.fetch_state(j, f64_spec(bs, jobs[k][1])) for j, k in zip(ids, keep)]
+    finally:
+        srv.close()
+
+    def drift(r):
+        return r.final_diag.mean_energy / r.baseline_diag.mean_energy - 1.0 if r.baseline_step else None
+
+    return {"reference": "the same jobs in f64 ExactMath through the server",
+            "jobs": [{"index": k, "edge": int(jobs[k][1].grid[0]), "steps": jobs[k][2],
+                      "drift_from_step": int(res[k].baseline_step), "energy_drift_f32": drift(res[k]),
+                      "energy_drift_f64": drift(r)} for k, r in zip(keep, r64)],
+            "blocks": field_errors([kept[k] for k in keep], ref)}
 
 
 def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     """The continuous-batching server on a stream of sweep jobs: Poisson
     arrivals at rate_hz (p50/p99 latency and aggregate throughput, which the
     arrival rate bounds), then the same jobs submitted at once (the server's
-    capacity), both against sequential plain solves of the same jobs."""
+    capacity), both against sequential plain solves of the same jobs (whole
+    jobs from step 0, as the server runs them)."""
     jobs = served_jobs(bs, n_jobs, rate_hz, seed)
     work = sum(spec.grid[0] * spec.grid[1] * spec.grid[2] * steps for (_, spec, steps) in jobs)
-    wall, lat, st, ok = serve(bs, jobs, burst=False)
+    # a sample for the accuracy check: the first job of each of four edges
+    keep, seen = [], set()
+    for k, (_, spec, _) in enumerate(jobs):
+        e = int(spec.grid[0])
+        if e not in seen and len(keep) < 4:
+            seen.add(e)
+            keep.append(k)
+    wall, res, st, kept = serve(bs, jobs, burst=False, keep=keep)
+    lat = np.array([r.latency_s for r in res])
     served = work / wall
-    bwall, blat, _, bok = serve(bs, jobs, burst=True)
+    bwall, bres, _, _ = serve(bs, jobs, burst=True)
+    blat = np.array([r.latency_s for r in bres])
     burst = work / bwall
-    # sequential plain solves of the first 8 of the same jobs, whole jobs
-    # from step 0 as the server runs them, rate extrapolated to the rest
+    ok = all(r.status == 0 for r in res) and all(r.status == 0 for r in bres)
     seq_t, seq_work = 0.0, 0
-    for (_, spec, steps) in jobs[:8]:
+    for (_, spec, steps) in jobs:
         sim = bs.Simulation(spec)
         sim.synchronize()
         t1 = time.perf_counter()
@@ -277,13 +386,18 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     log(f"bench: served {n_jobs} jobs at {rate_hz} Hz: wall {wall:.1f} s, {served / 1e9:.2f} G/s, "
         f"p50 {np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s; all at once: wall {bwall:.1f} s, "
         f"{burst / 1e9:.2f} G/s; sequential {sequential / 1e9:.2f} G/s")
+    acc = served_accuracy(bs, jobs, keep, kept, res) if len(kept) == len(keep) else None
+    if acc:
+        log("bench: served accuracy (sample edges " + ", ".join(str(j["edge"]) for j in acc["jobs"]) + "): rel L2 "
+            + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()))
     return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "budget_bytes": int(st.budget_bytes),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,
-            "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": ok and bok,
+            "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": ok,
             "burst": {"wall_s": bwall, "site_updates_per_s": burst, "speedup_vs_sequential": burst / sequential,
                       "p50_latency_s": float(np.percentile(blat, 50)),
-                      "p99_latency_s": float(np.percentile(blat, 99))}}
+                      "p99_latency_s": float(np.percentile(blat, 99))},
+            "accuracy": acc}
 
 
 def copy_calibration(nbytes=1 << 30, repeats=3):
@@ -389,8 +503,8 @@ def hbm_context(bs, achieved, traffic, avg_launch_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000,
-                    help="timed steps (the default covers nucleation and the whole transition)")
+    ap.add_argument("--steps", type=int, default=800,
+                    help="timed steps, after the box has been advanced past its last nucleation and the warm-up")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -420,9 +534,11 @@ def main():
     log(f"bench: {n}^3 f32 fused, eta={spec.model.eta:.4f}, device {bs.device_name()}")
     sim = bs.Simulation(spec)
     cells = sim.cell_count()
-    sim.advance(args.warmup)
+    settle = settle_steps(bs, sim)
+    start = settle + args.warmup  # first timed step
+    sim.advance(start)
     sim.synchronize()
-    log(f"bench: warmup {args.warmup} steps done")
+    log(f"bench: advanced past the last nucleation ({settle} steps) and {args.warmup} warm-up steps")
     # whole-job throughput, no per-launch events
     t0 = time.perf_counter()
     sim.advance(args.steps)
@@ -433,20 +549,29 @@ def main():
     # around every stage launch on the solver's stream (the kernel's cost
     # depends on the state through the clock the power limit allows)
     sim.reset()
-    sim.advance(args.warmup)
+    sim.advance(start)
     kernel_ms, _ = sim.advance_timed(args.steps)
-    log(f"bench: steps {args.warmup}..{args.warmup + args.steps}: wall {wall_s * 1e3:.1f} ms, stage kernels "
+    log(f"bench: steps {start}..{start + args.steps}: wall {wall_s * 1e3:.1f} ms, stage kernels "
         f"{kernel_ms:.1f} ms (second pass); broken {d.broken_fraction:.4f} max|v| {d.max_velocity:.4f} "
         f"nonfinite {d.nonfinite}")
-    # for scale: the first steps alone, while the box is still nearly uniform
-    sim.reset()
-    sim.advance(args.warmup)
-    sim.synchronize()
-    n_early = min(30, args.steps)
-    t0 = time.perf_counter()
-    sim.advance(n_early)
-    sim.synchronize()
-    early = cells * n_early / (time.perf_counter() - t0)
+
+    def window_rate(first, count):
+        sim.reset()
+        sim.advance(first)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        sim.advance(count)
+        sim.synchronize()
+        return cells * count / (time.perf_counter() - t1)
+
+    # for scale: the first steps of the run, while the box is still nearly
+    # uniform, and steps after the transition, the box fully broken
+    n_scale = 30
+    early = window_rate(args.warmup, n_scale)
+    late_first = max(1000, start + args.steps)
+    late = window_rate(late_first, n_scale)
+    log(f"bench: steps {args.warmup}..{args.warmup + n_scale}: {early / 1e9:.2f} G/s; "
+        f"steps {late_first}..{late_first + n_scale}: {late / 1e9:.2f} G/s")
     value = cells * args.steps / wall_s
     launches = 3 * args.steps
     avg_launch_s = kernel_ms / 1e3 / launches
@@ -479,15 +604,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"single periodic {n}^3 box, 32 bubbles (seeded centres, steps U[0,200]), "
-                               f"eta log-U[0.02,1] draw = {spec.model.eta:.4f}, SSP-RK3, fused stage kernel",
+                               f"eta log-U[0.02,1] draw = {spec.model.eta:.4f}, SSP-RK3, fused stage kernel; "
+                               f"advanced untimed past its last nucleation ({settle} steps) and {args.warmup} "
+                               f"warm-up steps, then steps {start}..{start + args.steps} timed",
                    "grid": n, "bubbles": 32, "precision": "f32", "math": "fast",
+                   "timed_steps": [start, start + args.steps],
                    "eos": args.eos if not eos else f"gamma-law {args.eos_gamma}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "accuracy": acc,
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
-        "first_steps": {"steps": n_early, "site_updates_per_s": early},
+        "first_steps": {"steps": [args.warmup, args.warmup + n_scale], "site_updates_per_s": early},
+        "late_steps": {"steps": [late_first, late_first + n_scale], "site_updates_per_s": late},
         "hbm_context": hbm,
         "batched_sweep": batch,
         "served": served,

@@ -22,8 +22,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# BUBBLESIM_LIB selects another build of the library (same ABI)
-LIB_PATH = os.environ.get("BUBBLESIM_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libbubblesim.so")
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbubblesim.so")
 CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "bubble-sim")
 
 BS_OK, BS_ERR_INVALID, BS_ERR_OOM, BS_ERR_DEVICE, BS_ERR_NONFINITE, BS_ERR_NOT_READY, BS_ERR_NOT_FOUND = range(7)
@@ -102,18 +101,23 @@ class CHbmRates(C.Structure):
     _fields_ = [("read_gbps", C.c_double), ("write_gbps", C.c_double), ("copy_gbps", C.c_double)]
 
 
+SERVER_NO_GRAPHS = 1
+
+
 class CServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("reserve_bytes", C.c_uint64), ("max_active", C.c_uint32),
-                ("chunk_steps", C.c_uint32), ("precision", C.c_int32), ("math", C.c_int32)]
+                ("chunk_steps", C.c_uint32), ("precision", C.c_int32), ("math", C.c_int32),
+                ("flags", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class CJob(C.Structure):
-    _fields_ = [("spec", CSpec), ("steps", C.c_uint64), ("keep_state", C.c_int32), ("checksum", C.c_uint32)]
+    _fields_ = [("spec", CSpec), ("steps", C.c_uint64), ("keep_state", C.c_int32), ("checksum", C.c_uint32),
+                ("init_field", C.c_void_p), ("init_fluid", C.c_void_p)]
 
 
 class CJobResult(C.Structure):
     _fields_ = [("job_id", C.c_uint64), ("steps", C.c_uint64), ("final_diag", CDiagnostics),
-                ("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("baseline_diag", CDiagnostics), ("baseline_step", C.c_uint64), ("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
                 ("latency_s", C.c_double), ("state_checksum", C.c_uint64), ("status", C.c_int32),
                 ("pad", C.c_uint32)]
 
@@ -122,7 +126,8 @@ class CServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
                 ("rejected", C.c_uint64), ("site_updates", C.c_uint64), ("ticks", C.c_uint64),
                 ("bytes_in_use", C.c_uint64), ("budget_bytes", C.c_uint64), ("active", C.c_uint32),
-                ("queued", C.c_uint32), ("busy_s", C.c_double), ("elapsed_s", C.c_double)]
+                ("queued", C.c_uint32), ("busy_s", C.c_double), ("elapsed_s", C.c_double),
+                ("failed", C.c_uint64), ("nonfinite", C.c_uint64)]
 
 
 _P = C.c_void_p
@@ -193,6 +198,8 @@ _SIGS = {
     "bs_batch_upload": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "bs_batch_steps": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "bs_batch_tile": (C.c_int, [_P, _u32p, _u32p]),
+    "bs_batch_plan": (C.c_int, [_P, _u32p]),
+    "bs_batch_graph_steps": (C.c_uint64, [_P]),
     "bs_server_create": (C.c_int, [C.POINTER(CServerConfig), C.POINTER(_P)]),
     "bs_server_submit": (C.c_int, [_P, C.POINTER(CJob), C.POINTER(C.c_uint64)]),
     "bs_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(CJobResult)]),
@@ -634,9 +641,20 @@ class Batch:
         return int(v.value)
 
     def tile(self):
+        """Tile shape of RK stages 2 and 3."""
         a, b = C.c_uint32(), C.c_uint32()
         _check(lib().bs_batch_tile(self._h, C.byref(a), C.byref(b)))
         return int(a.value), int(b.value)
+
+    def plan(self):
+        """{"stage1": (tx, ty, zc), "stages23": (tx, ty, zc)} chosen by the autotuner."""
+        out = (C.c_uint32 * 6)()
+        _check(lib().bs_batch_plan(self._h, out))
+        return {"stage1": tuple(out[0:3]), "stages23": tuple(out[3:6])}
+
+    def graph_steps(self) -> int:
+        """Steps advanced so far through replay of a captured hipGraph."""
+        return int(lib().bs_batch_graph_steps(self._h))
 
     def download(self, box: int):
         s = self.specs[box]
@@ -661,8 +679,8 @@ class Server:
     """Continuous batching of sweep jobs under an HBM budget."""
 
     def __init__(self, budget_bytes: int = 0, reserve_bytes: int = 16 << 30, max_active: int = 0,
-                 chunk_steps: int = 10, precision: int = PRECISION_F32, math: int = MATH_FAST):
-        cfg = CServerConfig(budget_bytes, reserve_bytes, max_active, chunk_steps, precision, math)
+                 chunk_steps: int = 10, precision: int = PRECISION_F32, math: int = MATH_FAST, flags: int = 0):
+        cfg = CServerConfig(budget_bytes, reserve_bytes, max_active, chunk_steps, precision, math, flags, 0)
         self._h = C.c_void_p()
         _check(lib().bs_server_create(C.byref(cfg), C.byref(self._h)))
         self.precision = precision
@@ -678,8 +696,19 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, spec: CSpec, steps: int, keep_state: bool = False, checksum: bool = False) -> int:
-        job = CJob(spec, steps, 1 if keep_state else 0, 1 if checksum else 0)
+    def submit(self, spec: CSpec, steps: int, keep_state: bool = False, checksum: bool = False,
+               init_state=None) -> int:
+        """Queue a job; init_state = (field, fluid) arrays starts it from that
+        state instead of the initialised lattice (copied here)."""
+        job = CJob(spec, steps, 1 if keep_state else 0, 1 if checksum else 0, None, None)
+        if init_state is not None:
+            real = np.float64 if self.precision == PRECISION_F64 else np.float32
+            f = np.ascontiguousarray(init_state[0], dtype=real)
+            u = np.ascontiguousarray(init_state[1], dtype=real)
+            n = int(spec.grid[0]) * int(spec.grid[1]) * int(spec.grid[2])
+            assert f.size == 2 * n and u.size == 4 * n
+            job.init_field = f.ctypes.data
+            job.init_fluid = u.ctypes.data
         jid = C.c_uint64()
         _check(lib().bs_server_submit(self._h, C.byref(job), C.byref(jid)))
         return int(jid.value)

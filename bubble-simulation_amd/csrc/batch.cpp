@@ -8,6 +8,10 @@
 
 using namespace bs;
 
+// steps captured per graph (bs_batch_advance replays it, and steps the
+// remainder eagerly)
+constexpr int kBatchGraphSteps = 25;
+
 struct bs_batch {
     int precision = BS_PRECISION_F32;
     int math = BS_MATH_FAST;
@@ -79,6 +83,8 @@ void bs_batch_destroy(bs_batch *b) {
     b->entries.clear();
     b->rs.release();
     b->eng.drop_graph();
+    b->eng.reap_graphs(true);
+    b->eng.stream = nullptr;  // destroyed below, before the engine
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -88,7 +94,7 @@ int bs_batch_advance(bs_batch *b, uint64_t n, int32_t use_graph) {
     int rc = batch_sync_progress(b);
     if (rc) return rc;
     b->progress_valid = false;
-    return b->eng.run(n, use_graph != 0, 25);
+    return b->eng.run(n, use_graph != 0, kBatchGraphSteps);
 }
 
 int bs_batch_synchronize(bs_batch *b) {
@@ -151,9 +157,21 @@ int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out) {
 
 int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty) {
     if (!b) return set_error(BS_ERR_INVALID, "null batch");
-    if (tx) *tx = (uint32_t)b->eng.tile.tx;
-    if (ty) *ty = (uint32_t)b->eng.tile.ty;
+    if (tx) *tx = (uint32_t)b->eng.plan[1].tile.tx;
+    if (ty) *ty = (uint32_t)b->eng.plan[1].tile.ty;
     return BS_OK;
 }
+
+int bs_batch_plan(const bs_batch *b, uint32_t out[6]) {
+    if (!b || !out) return set_error(BS_ERR_INVALID, "null argument");
+    for (int c = 0; c < 2; ++c) {
+        out[3 * c + 0] = (uint32_t)b->eng.plan[c].tile.tx;
+        out[3 * c + 1] = (uint32_t)b->eng.plan[c].tile.ty;
+        out[3 * c + 2] = (uint32_t)b->eng.plan[c].zc;
+    }
+    return BS_OK;
+}
+
+uint64_t bs_batch_graph_steps(const bs_batch *b) { return b ? b->eng.graph_replayed : 0; }
 
 }  // extern "C"

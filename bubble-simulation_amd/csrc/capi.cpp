@@ -43,6 +43,10 @@ size_t bs_struct_size(int which) {
     case 7: return sizeof(bs_model);
     case 8: return sizeof(bs_event);
     case 9: return sizeof(bs_derived);
+    case 10: return sizeof(bs_server_config);
+    case 11: return sizeof(bs_job);
+    case 12: return sizeof(bs_job_result);
+    case 13: return sizeof(bs_server_stats);
     default: return 0;
     }
 }

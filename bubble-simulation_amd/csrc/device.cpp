@@ -111,15 +111,19 @@ int default_zc(const uint32_t g[3], TileShape t) {
     return zc;
 }
 
-TileShape tile_from_env(TileShape dflt) {
+int tile_from_env(TileShape dflt, int math, int precision, TileShape *out) {
+    TileShape t = dflt;
     if (const char *e = std::getenv("BS_TILE")) {
         int a = 0, b = 0;
-        if (std::sscanf(e, "%dx%d", &a, &b) == 2) {
-            TileShape t{a, b};
-            if (tile_index(t) >= 0) return t;
-        }
+        if (std::sscanf(e, "%dx%d", &a, &b) != 2 || tile_index(TileShape{a, b}) < 0)
+            return set_error(BS_ERR_INVALID, std::string("BS_TILE=") + e + " is not an instantiated tile shape");
+        t = TileShape{a, b};
     }
-    return dflt;
+    if (!tile_built(t, math, precision))
+        return set_error(BS_ERR_INVALID, "tile " + std::to_string(t.tx) + "x" + std::to_string(t.ty) +
+                                             " carries two halo cells per thread and is built for exact arithmetic only");
+    *out = t;
+    return BS_OK;
 }
 
 int ReduceScratch::alloc() {
@@ -177,9 +181,11 @@ void diagnostics_from_result(const ReduceResult &r, uint64_t n_cells, bs_diagnos
 
 int download_state(BoxState &bx, hipStream_t stream, void *field, void *fluid) {
     const size_t r = real_size(bx.precision);
+    if (field)
+        BS_HIP_TRY(hipMemcpyAsync(field, bx.field[bx.cur], bx.n_cells * 2 * r, hipMemcpyDeviceToHost, stream));
+    if (fluid)
+        BS_HIP_TRY(hipMemcpyAsync(fluid, bx.fluid[bx.cur], bx.n_cells * 4 * r, hipMemcpyDeviceToHost, stream));
     BS_HIP_TRY(hipStreamSynchronize(stream));
-    if (field) BS_HIP_TRY(hipMemcpy(field, bx.field[bx.cur], bx.n_cells * 2 * r, hipMemcpyDeviceToHost));
-    if (fluid) BS_HIP_TRY(hipMemcpy(fluid, bx.fluid[bx.cur], bx.n_cells * 4 * r, hipMemcpyDeviceToHost));
     return BS_OK;
 }
 

@@ -100,7 +100,8 @@ int launch_vis(BoxState &bx, Half4 *out, hipStream_t stream);
 int state_checksum(BoxState &bx, hipStream_t stream, uint64_t *out);
 int download_state(BoxState &bx, hipStream_t stream, void *field, void *fluid);
 
-// tile shape selection: BS_TILE env ("32x16"), else the autotuned or default
-TileShape tile_from_env(TileShape dflt);
+// tile shape of a launch: BS_TILE ("32x16") when set, else `dflt`;
+// BS_ERR_INVALID when BS_TILE names a shape not built for this arithmetic
+int tile_from_env(TileShape dflt, int math, int precision, TileShape *out);
 
 }  // namespace bs

@@ -245,30 +245,19 @@ __device__ __forceinline__ global_t<char> *uniform_ptr(const void *p) {
     return reinterpret_cast<global_t<char> *>(((uint64_t)hi << 32) | lo);
 }
 
-// Element `idx` of a plane (the plane's base is the same in every lane): the
-// byte offset is formed in 32 bits (spec validation keeps an x-y plane below
-// 4 GiB).  Loads and stores move one dwordx2 / dwordx4 per element; the
-// streaming forms are non-temporal, for data used once per stage -- U^n and
-// the output slot -- so they do not evict from L2 the U_in planes that the
-// halo reads come back to (measured +3 % at 512^3).  Two implementations:
-// buffer resources (default) and, with -DBS_GLOBAL_PLANE_ACCESS, global
-// pointers with a scalar base (8 64-bit vector adds per plane; 0.5 % slower
-// at 512^3, 7 % with the gamma-law EOS).
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef double d4v __attribute__((ext_vector_type(4)));
-typedef double d2v __attribute__((ext_vector_type(2)));
-template <class R> struct VecT;
-template <> struct VecT<float> { typedef f4v v4; typedef f2v v2; };
-template <> struct VecT<double> { typedef d4v v4; typedef d2v v2; };
-
-#ifndef BS_GLOBAL_PLANE_ACCESS
-// Plane access through a raw buffer resource: the plane's base and size sit
-// in four scalar registers (rebuilt per plane by a few scalar instructions)
-// and each lane supplies only a 32-bit byte offset that is fixed for the
-// whole z-march, so no vector instruction is spent on addresses inside the
-// loop.  Offsets at or past the plane's size read zero and drop stores, the
-// hardware's range check.  Cache policy 2 is the non-temporal hint.
+// Element `idx` of a plane (the plane's base is the same in every lane),
+// through a raw buffer resource: the plane's base and size sit in four
+// scalar registers (rebuilt per plane by a few scalar instructions) and each
+// lane supplies only a 32-bit byte offset that is fixed for the whole
+// z-march, so no vector instruction is spent on addresses inside the loop
+// (a global pointer with a scalar base spent 8 64-bit vector adds per plane:
+// 0.5 % slower at 512^3, 7 % with the gamma-law EOS).  Spec validation keeps
+// an x-y plane below 4 GiB.  Offsets at or past the plane's size read zero
+// and drop stores, the hardware's range check.  Loads and stores move one
+// dwordx2 / dwordx4 per element; the streaming forms carry cache policy 2,
+// the non-temporal hint, for data used once per stage -- U^n and the output
+// slot -- so they do not evict from L2 the U_in planes that the halo reads
+// come back to (measured +3 % at 512^3).
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kRawBufferWord3 = 0x00020000;  // raw 32-bit data, no swizzle
@@ -319,39 +308,6 @@ template <class E> __device__ __forceinline__ E stream_load(const PlaneRef<E> &p
 template <class E> __device__ __forceinline__ void stream_store(const PlaneRef<E> &p, unsigned idx, const E &v) {
     buf_store<kCacheStream>(p, idx, v);
 }
-#else
-template <class E> struct PlaneRef {
-    const E *base;
-};
-template <class E> __device__ __forceinline__ PlaneRef<E> plane_ref(const E *base, unsigned) { return PlaneRef<E>{base}; }
-template <class T, class E> __device__ __forceinline__ global_t<T> *elem_ptr(const E *base, unsigned idx) {
-    return reinterpret_cast<global_t<T> *>(uniform_ptr(base) + idx * (unsigned)sizeof(E));
-}
-template <class R> __device__ __forceinline__ V4<R> plane_load(const PlaneRef<V4<R>> &p, unsigned idx) {
-    const typename VecT<R>::v4 v = *elem_ptr<const typename VecT<R>::v4>(p.base, idx);
-    return V4<R>{v.x, v.y, v.z, v.w};
-}
-template <class R> __device__ __forceinline__ V2<R> plane_load(const PlaneRef<V2<R>> &p, unsigned idx) {
-    const typename VecT<R>::v2 v = *elem_ptr<const typename VecT<R>::v2>(p.base, idx);
-    return V2<R>{v.x, v.y};
-}
-template <class R> __device__ __forceinline__ V4<R> stream_load(const PlaneRef<V4<R>> &p, unsigned idx) {
-    const typename VecT<R>::v4 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v4>(p.base, idx));
-    return V4<R>{v.x, v.y, v.z, v.w};
-}
-template <class R> __device__ __forceinline__ V2<R> stream_load(const PlaneRef<V2<R>> &p, unsigned idx) {
-    const typename VecT<R>::v2 v = __builtin_nontemporal_load(elem_ptr<const typename VecT<R>::v2>(p.base, idx));
-    return V2<R>{v.x, v.y};
-}
-template <class R> __device__ __forceinline__ void stream_store(const PlaneRef<V4<R>> &p, unsigned idx, const V4<R> &v) {
-    typename VecT<R>::v4 t = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v4>(p.base, idx));
-}
-template <class R> __device__ __forceinline__ void stream_store(const PlaneRef<V2<R>> &p, unsigned idx, const V2<R> &v) {
-    typename VecT<R>::v2 t = {v.x, v.y};
-    __builtin_nontemporal_store(t, elem_ptr<typename VecT<R>::v2>(p.base, idx));
-}
-#endif
 
 template <int TX, int TY, class M, class R, bool S0, class E>
 __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<R> &P,
@@ -644,18 +600,16 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 
 // Minimum waves per SIMD the register allocator must leave room for: the
 // fast f32 stage, with its march unrolled by three (so the column rings
-// advance by register renaming), needs 116-124 VGPRs, which 4 waves allow
+// advance by register renaming), needs 116-124 VGPRs on the tiles that
+// carry one halo item per thread (every fast tile), which 4 waves allow
 // without spilling (at 5 waves, 96 VGPRs, it spills); it runs about 0.5 %
 // faster than the rolled march at 5 waves.  The exact and f64
 // instantiations keep the compiler's choice.
 template <class M> struct Occupancy {
     static constexpr int waves = 1;
 };
-#ifndef BS_FAST_WAVES_PER_EU
-#define BS_FAST_WAVES_PER_EU 4
-#endif
 template <> struct Occupancy<FastMathF32> {
-    static constexpr int waves = BS_FAST_WAVES_PER_EU;
+    static constexpr int waves = 4;
 };
 
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with

@@ -184,34 +184,37 @@ hipError_t launch_stage_twopass<double>(int, const Box<double> &box, int stage, 
 //  Batch bookkeeping: device-driven nucleation and the per-step advance, so
 //  a captured step graph can be replayed without host involvement.
 // ---------------------------------------------------------------------------
-constexpr int kStampBlocksPerBox = 64;
+// Nucleation of a batch: the blocks walk the boxes in order and stamp the
+// events due before the current step of each box, every block taking its
+// share of that box's cells, so a step on which no box has an event due
+// costs one pass over the box table per block.
+constexpr int kStampBlocks = 256;
 
 template <class R>
 __global__ void __launch_bounds__(256) k_stamp_multi(Box<R> *boxes, const BoxEvents *events, int nboxes) {
-    const int bi = blockIdx.x / kStampBlocksPerBox;
-    const int part = blockIdx.x % kStampBlocksPerBox;
-    if (bi >= nboxes) return;
-    const BoxEvents &ev = events[bi];
-    if (!ev.active || ev.steps_done >= ev.steps_target) return;
-    if (ev.next >= ev.count || ev.step[ev.next] > ev.steps_done) return;
-    int last = ev.next;
-    while (last < ev.count && ev.step[last] <= ev.steps_done) ++last;
-    const Box<R> &b = boxes[bi];
-    const Params<R> &P = b.P;
-    V2<R> *field = b.field[b.cur];
-    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
-    for (int64_t i = (int64_t)part * 256 + threadIdx.x; i < n; i += (int64_t)kStampBlocksPerBox * 256) {
-        const int x = (int)(i % P.nx);
-        const int y = (int)((i / P.nx) % P.ny);
-        const int z = (int)(i / ((int64_t)P.nx * P.ny));
-        V2<R> f = field[i];
-        R phi = f.x;
-        for (int k = ev.next; k < last; ++k) {
-            const float *bb = ev.bubble + 4 * k;
-            const R prof = bubble_profile<R>(x, y, z, (R)bb[0], (R)bb[1], (R)bb[2], (R)bb[3], P);
-            phi = ExactMath<R>::max(phi, prof);
+    for (int bi = 0; bi < nboxes; ++bi) {
+        const BoxEvents &ev = events[bi];
+        if (!ev.active || ev.steps_done >= ev.steps_target) continue;
+        if (ev.next >= ev.count || ev.step[ev.next] > ev.steps_done) continue;
+        int last = ev.next;
+        while (last < ev.count && ev.step[last] <= ev.steps_done) ++last;
+        const Box<R> &b = boxes[bi];
+        const Params<R> &P = b.P;
+        V2<R> *field = b.field[b.cur];
+        const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kStampBlocks * 256) {
+            const int x = (int)(i % P.nx);
+            const int y = (int)((i / P.nx) % P.ny);
+            const int z = (int)(i / ((int64_t)P.nx * P.ny));
+            V2<R> f = field[i];
+            R phi = f.x;
+            for (int k = ev.next; k < last; ++k) {
+                const float *bb = ev.bubble + 4 * k;
+                const R prof = bubble_profile<R>(x, y, z, (R)bb[0], (R)bb[1], (R)bb[2], (R)bb[3], P);
+                phi = ExactMath<R>::max(phi, prof);
+            }
+            field[i] = V2<R>{phi, f.y};
         }
-        field[i] = V2<R>{phi, f.y};
     }
 }
 
@@ -226,7 +229,40 @@ template <class R> __global__ void k_advance_multi(Box<R> *boxes, BoxEvents *eve
 }
 
 template <class R> hipError_t launch_stamp_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st) {
-    hipLaunchKernelGGL(k_stamp_multi<R>, dim3(nboxes * kStampBlocksPerBox), dim3(256), 0, st, dboxes, devents, nboxes);
+    hipLaunchKernelGGL(k_stamp_multi<R>, dim3(kStampBlocks), dim3(256), 0, st, dboxes, devents, nboxes);
+    return hipGetLastError();
+}
+
+// Non-finite check of a set of boxes: kFiniteBlocksPerBox blocks per box
+// stream slot cur (24 B per cell in f32); any block that meets a NaN or
+// infinity writes the tick into the box's flag (the same value from every
+// writer, so the plain stores need no atomics).
+constexpr int kFiniteBlocksPerBox = 32;
+
+template <class R>
+__global__ void __launch_bounds__(256) k_finite_multi(const Box<R> *boxes, int nboxes, uint32_t *flags, uint32_t tick) {
+    const int bi = blockIdx.x / kFiniteBlocksPerBox;
+    const int part = blockIdx.x % kFiniteBlocksPerBox;
+    if (bi >= nboxes) return;
+    const Box<R> &b = boxes[bi];
+    const V2<R> *field = b.field[b.cur];
+    const V4<R> *fluid = b.fluid[b.cur];
+    const int64_t n = (int64_t)b.P.nx * b.P.ny * b.P.nz;
+    bool bad = false;
+    for (int64_t i = (int64_t)part * 256 + threadIdx.x; i < n; i += (int64_t)kFiniteBlocksPerBox * 256) {
+        const V2<R> f = field[i];
+        const V4<R> u = fluid[i];
+        const bool ok = isfinite(f.x) && isfinite(f.y) && isfinite(u.x) && isfinite(u.y) && isfinite(u.z) &&
+                        isfinite(u.w);
+        bad = bad || !ok;
+    }
+    if (bad) flags[bi] = tick;
+}
+
+template <class R>
+hipError_t launch_finite_multi(const Box<R> *dboxes, int nboxes, uint32_t *flags, uint32_t tick, hipStream_t st) {
+    hipLaunchKernelGGL(k_finite_multi<R>, dim3(nboxes * kFiniteBlocksPerBox), dim3(256), 0, st, dboxes, nboxes, flags,
+                       tick);
     return hipGetLastError();
 }
 template <class R> hipError_t launch_advance_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st) {
@@ -250,5 +286,7 @@ template hipError_t launch_stamp_multi<float>(Box<float> *, BoxEvents *, int, hi
 template hipError_t launch_stamp_multi<double>(Box<double> *, BoxEvents *, int, hipStream_t);
 template hipError_t launch_advance_multi<float>(Box<float> *, BoxEvents *, int, hipStream_t);
 template hipError_t launch_advance_multi<double>(Box<double> *, BoxEvents *, int, hipStream_t);
+template hipError_t launch_finite_multi<float>(const Box<float> *, int, uint32_t *, uint32_t, hipStream_t);
+template hipError_t launch_finite_multi<double>(const Box<double> *, int, uint32_t *, uint32_t, hipStream_t);
 
 }  // namespace bs

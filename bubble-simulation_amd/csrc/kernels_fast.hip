@@ -1,5 +1,6 @@
 // kernels_fast.hip -- FastMathF32 instantiations of the stage kernels.
-// Built with -ffp-contract=fast: the compiler fuses multiply-adds.
+// Built with -ffp-contract=fast: the compiler fuses multiply-adds.  Only the
+// tiles with one halo item per thread are built (tile_built in launch.h).
 #include "launch.h"
 
 namespace bs {
@@ -36,9 +37,7 @@ hipError_t launch_stage_fused_fast(const Box<float> &box, int stage, TileShape t
     case 1: return fused_one<64, 8>(box, stage, zc, st);
     case 2: return fused_one<32, 8>(box, stage, zc, st);
     case 3: return fused_one<16, 16>(box, stage, zc, st);
-    case 4: return fused_one<64, 4>(box, stage, zc, st);
     case 5: return fused_one<16, 8>(box, stage, zc, st);
-    case 6: return fused_one<32, 4>(box, stage, zc, st);
     case 7: return fused_one<16, 15>(box, stage, zc, st);
     default: return hipErrorInvalidValue;
     }
@@ -51,9 +50,7 @@ hipError_t launch_stage_fused_multi_fast(const Box<float> *dboxes, const BoxEven
     case 1: return fused_multi_one<64, 8>(dboxes, devents, dtasks, ntasks, stage, st);
     case 2: return fused_multi_one<32, 8>(dboxes, devents, dtasks, ntasks, stage, st);
     case 3: return fused_multi_one<16, 16>(dboxes, devents, dtasks, ntasks, stage, st);
-    case 4: return fused_multi_one<64, 4>(dboxes, devents, dtasks, ntasks, stage, st);
     case 5: return fused_multi_one<16, 8>(dboxes, devents, dtasks, ntasks, stage, st);
-    case 6: return fused_multi_one<32, 4>(dboxes, devents, dtasks, ntasks, stage, st);
     case 7: return fused_multi_one<16, 15>(dboxes, devents, dtasks, ntasks, stage, st);
     default: return hipErrorInvalidValue;
     }

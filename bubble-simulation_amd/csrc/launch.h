@@ -22,6 +22,16 @@ constexpr int kNumTiles = 8;
 constexpr TileShape kTiles[kNumTiles] = {{32, 16}, {64, 8}, {32, 8}, {16, 16}, {64, 4}, {16, 8}, {32, 4}, {16, 15}};
 int tile_index(TileShape t);  // -1 if not instantiated
 
+// Halo cells each thread of a TX x TY tile loads per plane: the 4 (TX + TY)
+// cells of the 2-cell cross halo spread over the TX * TY threads.
+constexpr int halo_items(TileShape t) { return (4 * (t.tx + t.ty) + t.tx * t.ty - 1) / (t.tx * t.ty); }
+// The FastMath kernel is built for the tiles with one halo item per thread:
+// a second item (64x4, 32x4) needs 128 VGPRs and spills at the 4 waves per
+// SIMD the fast kernel is compiled for.  The exact kernels take every tile.
+constexpr bool tile_built(TileShape t, int math, int precision) {
+    return precision == BS_PRECISION_F64 || math == BS_MATH_EXACT || halo_items(t) == 1;
+}
+
 template <class R> hipError_t launch_init(V2<R> *field, V4<R> *fluid, int64_t n, R e_ref, hipStream_t st);
 template <class R> hipError_t launch_stamp(V2<R> *field, const Params<R> &P, const StampBatch<R> &b, hipStream_t st);
 template <class R>
@@ -51,5 +61,8 @@ hipError_t launch_stage_twopass_fast(const Box<float> &box, int stage, V4<float>
 template <class R>
 hipError_t launch_stamp_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
 template <class R> hipError_t launch_advance_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
+// flags[box] = tick for every box whose slot cur holds a non-finite value
+template <class R>
+hipError_t launch_finite_multi(const Box<R> *dboxes, int nboxes, uint32_t *flags, uint32_t tick, hipStream_t st);
 
 }  // namespace bs

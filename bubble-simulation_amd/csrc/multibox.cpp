@@ -1,6 +1,7 @@
 // multibox.cpp -- Entry / Engine of the batched and served paths.
 #include "multibox.h"
 
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -23,15 +24,15 @@ int Entry::create(const bs_spec &s, hipStream_t stream, bool async) {
     rc = bx.init(stream);
     if (rc) return rc;
     const size_t ne = schedule.size();
-    std::vector<uint64_t> steps(ne);
-    std::vector<float> bub(4 * ne);
+    h_steps.resize(ne);
+    h_bubbles.resize(4 * ne);
     const float dt = model_dt(s.model);
     for (size_t k = 0; k < ne; ++k) {
-        steps[k] = event_step(schedule[k].time, dt);
-        bub[4 * k + 0] = schedule[k].pos[0];
-        bub[4 * k + 1] = schedule[k].pos[1];
-        bub[4 * k + 2] = schedule[k].pos[2];
-        bub[4 * k + 3] = schedule[k].radius;
+        h_steps[k] = event_step(schedule[k].time, dt);
+        h_bubbles[4 * k + 0] = schedule[k].pos[0];
+        h_bubbles[4 * k + 1] = schedule[k].pos[1];
+        h_bubbles[4 * k + 2] = schedule[k].pos[2];
+        h_bubbles[4 * k + 3] = schedule[k].radius;
     }
     if (ne) {
         if (async) {
@@ -41,10 +42,9 @@ int Entry::create(const bs_spec &s, hipStream_t stream, bool async) {
             BS_HIP_TRY(hipMalloc((void **)&d_steps, ne * sizeof(uint64_t)));
             BS_HIP_TRY(hipMalloc((void **)&d_bubbles, ne * 4 * sizeof(float)));
         }
-        BS_HIP_TRY(hipMemcpyAsync(d_steps, steps.data(), ne * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
-        BS_HIP_TRY(hipMemcpyAsync(d_bubbles, bub.data(), ne * 4 * sizeof(float), hipMemcpyHostToDevice, stream));
-        // the host vectors must outlive the copies
-        BS_HIP_TRY(hipStreamSynchronize(stream));
+        // the host tables live as long as the entry, so the copies need no wait
+        BS_HIP_TRY(hipMemcpyAsync(d_steps, h_steps.data(), ne * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+        BS_HIP_TRY(hipMemcpyAsync(d_bubbles, h_bubbles.data(), ne * 4 * sizeof(float), hipMemcpyHostToDevice, stream));
     }
     steps_done = 0;
     next_event = 0;
@@ -65,35 +65,79 @@ uint64_t Entry::device_bytes() const {
            schedule.size() * 20;
 }
 
+void Entry::host_advance(uint64_t n) {
+    for (uint64_t i = 0; i < n && steps_done < steps_target; ++i) {
+        while (next_event < (int)h_steps.size() && h_steps[next_event] <= steps_done) ++next_event;
+        steps_done += 1;
+        bx.cur = (bx.cur + 1) % 3;
+    }
+}
+
 Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
     drop_graph();
+    reap_graphs(true);
     if (d_boxes) (void)hipFree(d_boxes);
     if (d_events) (void)hipFree(d_events);
-    if (d_tasks) (void)hipFree(d_tasks);
+    if (d_flags) (void)hipFree(d_flags);
+    for (Plan &p : plan)
+        if (p.d_tasks) (void)hipFree(p.d_tasks);
+    for (int i = 0; i < 2; ++i) {
+        if (stage_buf[i]) (void)hipHostFree(stage_buf[i]);
+        if (stage_done[i]) (void)hipEventDestroy(stage_done[i]);
+    }
 }
 
 void Engine::drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    if (gexec || graph) {
+        // a launch of this graph may still be queued: destroy it only once
+        // the stream has passed this point
+        hipEvent_t ev = nullptr;
+        if (stream && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+            hipEventRecord(ev, stream) == hipSuccess) {
+            graveyard.push_back(Dead{graph, gexec, ev});
+        } else {
+            if (ev) (void)hipEventDestroy(ev);
+            (void)hipGetLastError();
+            if (stream) (void)hipStreamSynchronize(stream);
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    }
     gexec = nullptr;
     graph = nullptr;
     graph_steps = 0;
+    reap_graphs(false);
 }
 
-template <class R> static void fill_boxes(const std::vector<Entry *> &es, std::vector<unsigned char> &buf) {
-    buf.resize(es.size() * sizeof(Box<R>));
-    Box<R> *b = (Box<R> *)buf.data();
+void Engine::reap_graphs(bool wait) {
+    for (size_t i = 0; i < graveyard.size();) {
+        Dead &d = graveyard[i];
+        hipError_t q = wait ? hipEventSynchronize(d.after) : hipEventQuery(d.after);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        if (d.x) (void)hipGraphExecDestroy(d.x);
+        if (d.g) (void)hipGraphDestroy(d.g);
+        (void)hipEventDestroy(d.after);
+        graveyard.erase(graveyard.begin() + (long)i);
+    }
+}
+
+template <class R> static void fill_boxes(const std::vector<Entry *> &es, unsigned char *out) {
+    Box<R> *b = (Box<R> *)out;
     for (size_t i = 0; i < es.size(); ++i) b[i] = es[i]->bx.box<R>();
 }
 
-int Engine::upload() {
-    drop_graph();
-    const size_t n = entries.size();
-    const size_t box_bytes = precision == BS_PRECISION_F64 ? sizeof(Box<double>) : sizeof(Box<float>);
-    // tasks
-    std::vector<Task> tasks;
-    for (size_t i = 0; i < n; ++i) {
-        const uint32_t *g = entries[i]->spec.grid;
+// tasks of one stage class: (box, x0, y0, zb, ze) over every box, padded
+// with empty tasks to a multiple of 8, so that the kernel can hand each XCD
+// a contiguous run of tiles (blocks go round-robin over the 8 XCDs;
+// neighbouring tiles then share that XCD's L2)
+static void build_tasks(const std::vector<Entry *> &es, TileShape tile, int zc, std::vector<Task> &tasks) {
+    tasks.clear();
+    for (size_t i = 0; i < es.size(); ++i) {
+        const uint32_t *g = es[i]->spec.grid;
         for (uint32_t zb = 0; zb < g[2]; zb += (uint32_t)zc)
             for (uint32_t y0 = 0; y0 < g[1]; y0 += (uint32_t)tile.ty)
                 for (uint32_t x0 = 0; x0 < g[0]; x0 += (uint32_t)tile.tx) {
@@ -107,36 +151,72 @@ int Engine::upload() {
                     tasks.push_back(t);
                 }
     }
-    // pad to a multiple of 8 with empty tasks, so that the kernel can hand
-    // each XCD a contiguous run of tiles (blocks go round-robin over the 8
-    // XCDs; neighbouring tiles then share that XCD's L2)
     while (!tasks.empty() && tasks.size() % 8 != 0) {
         Task t;
         std::memset(&t, 0, sizeof t);
         tasks.push_back(t);
     }
-    ntasks = (int)tasks.size();
-    if (n > cap_boxes) {
+}
+
+int Engine::upload() {
+    drop_graph();
+    const size_t n = entries.size();
+    const size_t box_bytes = precision == BS_PRECISION_F64 ? sizeof(Box<double>) : sizeof(Box<float>);
+    std::vector<Task> tasks[2];
+    for (int p = 0; p < 2; ++p) {
+        build_tasks(entries, plan[p].tile, plan[p].zc, tasks[p]);
+        plan[p].ntasks = (int)tasks[p].size();
+    }
+    // device arrays grow rarely; a launch still queued may read the old
+    // ones, so the stream is drained before they are freed
+    const bool grow_boxes = n > cap_boxes;
+    const bool grow_tasks = (size_t)plan[0].ntasks > plan[0].cap || (size_t)plan[1].ntasks > plan[1].cap;
+    if (grow_boxes || grow_tasks) BS_HIP_TRY(hipStreamSynchronize(stream));
+    if (grow_boxes) {
         if (d_boxes) (void)hipFree(d_boxes);
         if (d_events) (void)hipFree(d_events);
+        if (d_flags) (void)hipFree(d_flags);
         d_boxes = nullptr;
         d_events = nullptr;
+        d_flags = nullptr;
+        cap_boxes = 0;
         const size_t cap = std::max<size_t>(n, 16);
         BS_HIP_TRY(hipMalloc(&d_boxes, cap * sizeof(Box<double>)));
         BS_HIP_TRY(hipMalloc((void **)&d_events, cap * sizeof(BoxEvents)));
+        BS_HIP_TRY(hipMalloc((void **)&d_flags, cap * sizeof(uint32_t)));
+        BS_HIP_TRY(hipMemsetAsync(d_flags, 0xff, cap * sizeof(uint32_t), stream));  // no tick matches
         cap_boxes = cap;
     }
-    if ((size_t)ntasks > cap_tasks) {
-        if (d_tasks) (void)hipFree(d_tasks);
-        d_tasks = nullptr;
-        const size_t cap = std::max<size_t>((size_t)ntasks, 256);
-        BS_HIP_TRY(hipMalloc((void **)&d_tasks, cap * sizeof(Task)));
-        cap_tasks = cap;
+    for (Plan &p : plan) {
+        if ((size_t)p.ntasks <= p.cap) continue;
+        if (p.d_tasks) (void)hipFree(p.d_tasks);
+        p.d_tasks = nullptr;
+        p.cap = 0;
+        const size_t cap = std::max<size_t>((size_t)p.ntasks, 256);
+        BS_HIP_TRY(hipMalloc((void **)&p.d_tasks, cap * sizeof(Task)));
+        p.cap = cap;
     }
-    std::vector<unsigned char> boxes;
-    if (precision == BS_PRECISION_F64) fill_boxes<double>(entries, boxes);
-    else fill_boxes<float>(entries, boxes);
-    std::vector<BoxEvents> ev(n);
+    // stage everything in pinned memory: [boxes | events | tasks 0 | tasks 1]
+    const size_t off_ev = (n * box_bytes + 255) & ~(size_t)255;
+    const size_t off_t0 = (off_ev + n * sizeof(BoxEvents) + 255) & ~(size_t)255;
+    const size_t off_t1 = (off_t0 + tasks[0].size() * sizeof(Task) + 255) & ~(size_t)255;
+    const size_t total = off_t1 + tasks[1].size() * sizeof(Task);
+    const int k = stage_next;
+    stage_next ^= 1;
+    if (!stage_done[k]) BS_HIP_TRY(hipEventCreateWithFlags(&stage_done[k], hipEventDisableTiming));
+    else BS_HIP_TRY(hipEventSynchronize(stage_done[k]));  // the copies out of this buffer have run
+    if (total > stage_cap[k]) {
+        if (stage_buf[k]) (void)hipHostFree(stage_buf[k]);
+        stage_buf[k] = nullptr;
+        stage_cap[k] = 0;
+        const size_t cap = std::max<size_t>(total * 2, 1 << 16);
+        BS_HIP_TRY(hipHostMalloc(&stage_buf[k], cap, hipHostMallocDefault));
+        stage_cap[k] = cap;
+    }
+    unsigned char *buf = (unsigned char *)stage_buf[k];
+    if (precision == BS_PRECISION_F64) fill_boxes<double>(entries, buf);
+    else fill_boxes<float>(entries, buf);
+    BoxEvents *ev = (BoxEvents *)(buf + off_ev);
     for (size_t i = 0; i < n; ++i) {
         Entry *e = entries[i];
         std::memset(&ev[i], 0, sizeof(BoxEvents));
@@ -148,13 +228,19 @@ int Engine::upload() {
         ev[i].steps_target = e->steps_target;
         ev[i].active = 1;
     }
+    if (!tasks[0].empty()) std::memcpy(buf + off_t0, tasks[0].data(), tasks[0].size() * sizeof(Task));
+    if (!tasks[1].empty()) std::memcpy(buf + off_t1, tasks[1].data(), tasks[1].size() * sizeof(Task));
     if (n) {
-        BS_HIP_TRY(hipMemcpyAsync(d_boxes, boxes.data(), n * box_bytes, hipMemcpyHostToDevice, stream));
-        BS_HIP_TRY(hipMemcpyAsync(d_events, ev.data(), n * sizeof(BoxEvents), hipMemcpyHostToDevice, stream));
+        BS_HIP_TRY(hipMemcpyAsync(d_boxes, buf, n * box_bytes, hipMemcpyHostToDevice, stream));
+        BS_HIP_TRY(hipMemcpyAsync(d_events, buf + off_ev, n * sizeof(BoxEvents), hipMemcpyHostToDevice, stream));
     }
-    if (ntasks)
-        BS_HIP_TRY(hipMemcpyAsync(d_tasks, tasks.data(), (size_t)ntasks * sizeof(Task), hipMemcpyHostToDevice, stream));
-    BS_HIP_TRY(hipStreamSynchronize(stream));
+    if (!tasks[0].empty())
+        BS_HIP_TRY(hipMemcpyAsync(plan[0].d_tasks, buf + off_t0, tasks[0].size() * sizeof(Task),
+                                  hipMemcpyHostToDevice, stream));
+    if (!tasks[1].empty())
+        BS_HIP_TRY(hipMemcpyAsync(plan[1].d_tasks, buf + off_t1, tasks[1].size() * sizeof(Task),
+                                  hipMemcpyHostToDevice, stream));
+    BS_HIP_TRY(hipEventRecord(stage_done[k], stream));
     return BS_OK;
 }
 
@@ -178,28 +264,26 @@ int Engine::download_progress() {
     return BS_OK;
 }
 
-int Engine::step_eager() {
-    const int n = (int)entries.size();
-    if (!n) return BS_OK;
-    if (precision == BS_PRECISION_F64) {
-        Box<double> *db = (Box<double> *)d_boxes;
-        BS_HIP_TRY(launch_stamp_multi<double>(db, d_events, n, stream));
-        for (int s = 0; s < 3; ++s)
-            BS_HIP_TRY(launch_stage_fused_multi<double>(math, db, d_events, d_tasks, ntasks, s, tile, stream));
-        BS_HIP_TRY(launch_advance_multi<double>(db, d_events, n, stream));
-    } else {
-        Box<float> *db = (Box<float> *)d_boxes;
-        BS_HIP_TRY(launch_stamp_multi<float>(db, d_events, n, stream));
-        for (int s = 0; s < 3; ++s)
-            BS_HIP_TRY(launch_stage_fused_multi<float>(math, db, d_events, d_tasks, ntasks, s, tile, stream));
-        BS_HIP_TRY(launch_advance_multi<float>(db, d_events, n, stream));
+template <class R> static int step_eager_t(Engine &e) {
+    const int n = (int)e.entries.size();
+    Box<R> *db = (Box<R> *)e.d_boxes;
+    BS_HIP_TRY(launch_stamp_multi<R>(db, e.d_events, n, e.stream));
+    for (int s = 0; s < 3; ++s) {
+        const Plan &p = e.plan[s == 0 ? 0 : 1];
+        BS_HIP_TRY(launch_stage_fused_multi<R>(e.math, db, e.d_events, p.d_tasks, p.ntasks, s, p.tile, e.stream));
     }
+    BS_HIP_TRY(launch_advance_multi<R>(db, e.d_events, n, e.stream));
     return BS_OK;
+}
+
+int Engine::step_eager() {
+    if (entries.empty()) return BS_OK;
+    return precision == BS_PRECISION_F64 ? step_eager_t<double>(*this) : step_eager_t<float>(*this);
 }
 
 int Engine::run(uint64_t n, bool use_graph, int graph_chunk) {
     if (entries.empty() || n == 0) return BS_OK;
-    if (!use_graph || graph_chunk <= 1 || n < (uint64_t)graph_chunk) {
+    if (!use_graphs || !use_graph || graph_chunk <= 1 || n < (uint64_t)graph_chunk) {
         for (uint64_t i = 0; i < n; ++i) {
             int rc = step_eager();
             if (rc) return rc;
@@ -208,7 +292,9 @@ int Engine::run(uint64_t n, bool use_graph, int graph_chunk) {
     }
     if (!gexec || graph_steps != graph_chunk) {
         drop_graph();
-        BS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        // relaxed: the capture holds kernel launches only, and other threads
+        // (a server's retire thread) keep using their own streams meanwhile
+        BS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = BS_OK;
         for (int i = 0; i < graph_chunk && !rc; ++i) rc = step_eager();
         hipGraph_t g = nullptr;
@@ -217,15 +303,24 @@ int Engine::run(uint64_t n, bool use_graph, int graph_chunk) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
         }
-        if (e != hipSuccess) return set_error(BS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_error(BS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        }
+        e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            gexec = nullptr;
+            return set_error(BS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
         graph = g;
-        BS_HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         graph_steps = graph_chunk;
     }
     uint64_t done = 0;
     while (n - done >= (uint64_t)graph_steps) {
         BS_HIP_TRY(hipGraphLaunch(gexec, stream));
         done += (uint64_t)graph_steps;
+        graph_replayed += (uint64_t)graph_steps;
     }
     for (; done < n; ++done) {
         int rc = step_eager();
@@ -234,44 +329,68 @@ int Engine::run(uint64_t n, bool use_graph, int graph_chunk) {
     return BS_OK;
 }
 
-int Engine::time_stage0(float *ms) {
-    hipEvent_t a, b;
-    BS_HIP_TRY(hipEventCreate(&a));
-    BS_HIP_TRY(hipEventCreate(&b));
-    int rc = BS_OK;
+int Engine::time_stages(float ms[2]) {
+    hipEvent_t ev[4];
+    for (auto &e : ev) BS_HIP_TRY(hipEventCreate(&e));
     hipError_t e = hipSuccess;
-    // stage 0 reads slot cur and writes the next slot: repeating it leaves
-    // U^n untouched
-    auto launch = [&]() -> hipError_t {
+    // stage 1 reads slot cur and writes the next slot, stages 2 and 3 read
+    // those and write the slots after cur: repeating them leaves U^n untouched
+    auto launch = [&](int s) -> hipError_t {
+        const Plan &p = plan[s == 0 ? 0 : 1];
         if (precision == BS_PRECISION_F64)
-            return launch_stage_fused_multi<double>(math, (Box<double> *)d_boxes, d_events, d_tasks, ntasks, 0, tile, stream);
-        return launch_stage_fused_multi<float>(math, (Box<float> *)d_boxes, d_events, d_tasks, ntasks, 0, tile, stream);
+            return launch_stage_fused_multi<double>(math, (Box<double> *)d_boxes, d_events, p.d_tasks, p.ntasks, s,
+                                                    p.tile, stream);
+        return launch_stage_fused_multi<float>(math, (Box<float> *)d_boxes, d_events, p.d_tasks, p.ntasks, s, p.tile,
+                                               stream);
     };
-    e = launch();  // warm
-    if (e == hipSuccess) e = hipEventRecord(a, stream);
-    for (int r = 0; r < 3 && e == hipSuccess; ++r) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(b, stream);
-    if (e == hipSuccess) e = hipEventSynchronize(b);
-    float t = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
-    if (e != hipSuccess) rc = set_error(BS_ERR_DEVICE, std::string("autotune timing: ") + hipGetErrorString(e));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    *ms = t / 3.0f;
-    return rc;
+    constexpr int kReps = 3;
+    for (int s = 0; s < 3 && e == hipSuccess; ++s) e = launch(s);  // warm
+    if (e == hipSuccess) e = hipEventRecord(ev[0], stream);
+    for (int r = 0; r < kReps && e == hipSuccess; ++r) e = launch(0);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], stream);
+    for (int r = 0; r < kReps && e == hipSuccess; ++r) {
+        e = launch(1);
+        if (e == hipSuccess) e = launch(2);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[3], stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+    float t0 = 0.0f, t1 = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t0, ev[0], ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t1, ev[2], ev[3]);
+    for (auto &x : ev) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return set_error(BS_ERR_DEVICE, std::string("autotune timing: ") + hipGetErrorString(e));
+    ms[0] = t0 / kReps;
+    ms[1] = t1 / (2 * kReps);
+    return BS_OK;
+}
+
+int Engine::check_finite(uint32_t tick) {
+    const int n = (int)entries.size();
+    if (!n) return BS_OK;
+    if (precision == BS_PRECISION_F64)
+        BS_HIP_TRY(launch_finite_multi<double>((const Box<double> *)d_boxes, n, d_flags, tick, stream));
+    else
+        BS_HIP_TRY(launch_finite_multi<float>((const Box<float> *)d_boxes, n, d_flags, tick, stream));
+    return BS_OK;
 }
 
 int autotune(Engine &eng) {
-    if (const char *e = std::getenv("BS_TILE")) {
-        int a = 0, b = 0;
-        if (std::sscanf(e, "%dx%d", &a, &b) == 2 && tile_index(TileShape{a, b}) >= 0) {
-            eng.tile = TileShape{a, b};
-            if (const char *z = std::getenv("BS_ZC")) eng.zc = std::max(1, std::atoi(z));
-            return eng.upload();
+    const int math = eng.precision == BS_PRECISION_F64 ? BS_MATH_EXACT : eng.math;
+    if (std::getenv("BS_TILE")) {
+        TileShape t;
+        int rc = tile_from_env(TileShape{16, 16}, math, eng.precision, &t);
+        if (rc) return rc;
+        int zc = 32;
+        if (const char *z = std::getenv("BS_ZC")) zc = std::max(1, std::atoi(z));
+        for (Plan &p : eng.plan) {
+            p.tile = t;
+            p.zc = zc;
         }
+        return eng.upload();
     }
     static std::mutex mu;
-    static std::map<std::string, std::pair<int, int>> cache;
+    static std::map<std::string, std::array<int, 4>> cache;
     std::string key = std::to_string(eng.precision) + ":" + std::to_string(eng.math);
     for (Entry *e : eng.entries)
         key += ":" + std::to_string(e->spec.grid[0]) + "x" + std::to_string(e->spec.grid[1]) + "x" +
@@ -280,37 +399,57 @@ int autotune(Engine &eng) {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            eng.tile = kTiles[it->second.first];
-            eng.zc = it->second.second;
+            eng.plan[0].tile = kTiles[it->second[0]];
+            eng.plan[0].zc = it->second[1];
+            eng.plan[1].tile = kTiles[it->second[2]];
+            eng.plan[1].zc = it->second[3];
             return eng.upload();
         }
     }
-    float best = 1e30f;
-    int best_t = 0, best_zc = 32;
+    // time on a state with structure: every box's events stamped into slot
+    // cur (the stage kernels' clock depends on the data they stream); the
+    // boxes are initialised again afterwards
+    for (Entry *e : eng.entries) {
+        int rc = stamp_bubbles(e->bx, e->schedule, eng.stream);
+        if (rc) return rc;
+    }
+    float best[2] = {1e30f, 1e30f};
+    int best_t[2] = {3, 2}, best_zc[2] = {32, 32};
     const int zcs[3] = {16, 32, 64};
     for (int ti = 0; ti < kNumTiles; ++ti) {
+        if (!tile_built(kTiles[ti], math, eng.precision)) continue;
         if (eng.precision == BS_PRECISION_F64 && kTiles[ti].tx * kTiles[ti].ty > 256) continue;
         for (int zc : zcs) {
-            eng.tile = kTiles[ti];
-            eng.zc = zc;
+            for (Plan &p : eng.plan) {
+                p.tile = kTiles[ti];
+                p.zc = zc;
+            }
             int rc = eng.upload();
             if (rc) return rc;
-            float ms = 0.0f;
-            rc = eng.time_stage0(&ms);
+            float ms[2] = {0.0f, 0.0f};
+            rc = eng.time_stages(ms);
             if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                best_t = ti;
-                best_zc = zc;
+            for (int c = 0; c < 2; ++c) {
+                if (ms[c] < best[c]) {
+                    best[c] = ms[c];
+                    best_t[c] = ti;
+                    best_zc[c] = zc;
+                }
             }
         }
     }
+    for (Entry *e : eng.entries) {
+        int rc = e->bx.init(eng.stream);
+        if (rc) return rc;
+    }
     {
         std::lock_guard<std::mutex> lk(mu);
-        cache[key] = {best_t, best_zc};
+        cache[key] = {best_t[0], best_zc[0], best_t[1], best_zc[1]};
     }
-    eng.tile = kTiles[best_t];
-    eng.zc = best_zc;
+    for (int c = 0; c < 2; ++c) {
+        eng.plan[c].tile = kTiles[best_t[c]];
+        eng.plan[c].zc = best_zc[c];
+    }
     return eng.upload();
 }
 

@@ -15,40 +15,73 @@ struct Entry {
     bs_spec spec{};
     BoxState bx;
     std::vector<bs_event> schedule;
-    uint64_t *d_steps = nullptr;   // event stamping steps
-    float *d_bubbles = nullptr;    // 4 floats per event
+    std::vector<uint64_t> h_steps;  // stamping step of each event (host copy)
+    std::vector<float> h_bubbles;   // 4 floats per event (host copy, alive until the upload has run)
+    uint64_t *d_steps = nullptr;    // event stamping steps
+    float *d_bubbles = nullptr;     // 4 floats per event
     uint64_t steps_target = UINT64_MAX;
     // host mirrors of the device progress
     uint64_t steps_done = 0;
     int next_event = 0;
 
-    // allocate slots + event tables, init (async on `stream`)
+    // allocate slots + event tables, init; stream-ordered when `async`, and
+    // then nothing here waits for the stream
     int create(const bs_spec &s, hipStream_t stream, bool async);
     void release(hipStream_t stream);
     uint64_t device_bytes() const;
+    // the step before which the last event is stamped (0 without events)
+    uint64_t last_event_step() const { return h_steps.empty() ? 0 : h_steps.back(); }
+    // mirror of n device steps (k_advance_multi): events due at a step are
+    // passed, then the step is counted and the slot ring turns
+    void host_advance(uint64_t n);
+};
+
+// Tile shape and z chunk of one class of stage launches, with its task
+// table: the first RK stage (reads U_in only) and the other two (stream U^n
+// as well) are bound differently and may take different tiles.
+struct Plan {
+    TileShape tile{16, 16};
+    int zc = 32;
+    Task *d_tasks = nullptr;
+    int ntasks = 0;
+    size_t cap = 0;
 };
 
 // The launch machinery over a set of entries.
 struct Engine {
     int precision = BS_PRECISION_F32;
     int math = BS_MATH_FAST;
-    TileShape tile{32, 16};
-    int zc = 32;
+    Plan plan[2];  // [0]: RK stage 1, [1]: RK stages 2 and 3
     hipStream_t stream = nullptr;
 
     std::vector<Entry *> entries;   // the current set (order = device index)
     void *d_boxes = nullptr;        // Box<R>[n]
     BoxEvents *d_events = nullptr;  // BoxEvents[n]
-    Task *d_tasks = nullptr;
-    int ntasks = 0;
-    size_t cap_boxes = 0, cap_tasks = 0;
+    uint32_t *d_flags = nullptr;    // per box: tick of the last non-finite check that failed
+    size_t cap_boxes = 0;
+    // pinned staging of the uploads, two buffers used in turn; an upload
+    // waits only for the copies out of the buffer it reuses
+    void *stage_buf[2] = {nullptr, nullptr};
+    size_t stage_cap[2] = {0, 0};
+    hipEvent_t stage_done[2] = {nullptr, nullptr};
+    int stage_next = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
+    uint64_t graph_replayed = 0;  // steps run through graph replay
+    bool use_graphs = true;
+    // dropped graphs, destroyed once the stream has passed their last launch
+    struct Dead {
+        hipGraph_t g;
+        hipGraphExec_t x;
+        hipEvent_t after;
+    };
+    std::vector<Dead> graveyard;
 
     ~Engine();
     // upload the current host mirrors of `entries` (progress, cur) and the
-    // task table; invalidates a captured graph
+    // task tables (stream-ordered; the host does not wait for the stream);
+    // invalidates a captured graph
     int upload();
     // read device progress (steps_done, next, cur) back into the entries
     int download_progress();
@@ -56,12 +89,21 @@ struct Engine {
     int step_eager();
     // n steps, through a captured graph of `graph_chunk` steps when use_graph
     int run(uint64_t n, bool use_graph, int graph_chunk);
-    // time one stage-0 launch of the current set (idempotent) in ms
-    int time_stage0(float *ms);
+    // mean time of one launch of each stage class over the current set, in
+    // ms (stage 1 alone; stages 2 and 3 as a pair, per launch)
+    int time_stages(float ms[2]);
+    // flag every box of the set whose slot cur holds a non-finite value:
+    // d_flags[box] = tick (never cleared: a flag is current when it equals
+    // the tick of the check)
+    int check_finite(uint32_t tick);
     void drop_graph();
+    void reap_graphs(bool wait);
 };
 
-// pick the tile shape and z chunk minimising the stage time of the set
+// pick the tile shape and z chunk of each stage class, minimising the stage
+// times of the set; the set's entries must be freshly initialised (slot cur
+// is stamped with every event of its box for the timing and initialised
+// again afterwards)
 int autotune(Engine &eng);
 
 }  // namespace bs

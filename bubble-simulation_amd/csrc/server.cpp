@@ -1,17 +1,25 @@
 // server.cpp -- bs_server: continuous batching of sweep jobs.
 //
-// A worker thread owns the device.  Each scheduling tick it
+// Two host threads share the device.  The worker owns the stepping stream;
+// each scheduling tick it
 //   1. admits queued jobs in arrival order while their lattice fits the HBM
 //      budget (and the active-slot cap), allocating their state
 //      stream-ordered and initialising it on the device;
 //   2. advances every active job by `chunk_steps` steps with one fused launch
 //      per RK stage across all of them (a captured hipGraph when the active
 //      set did not change since the previous tick);
-//   3. retires jobs that reached their step count: final diagnostics, state
-//      checksum (and optionally the state itself) are kept as the result and
-//      the device memory is returned to the pool.
+//   3. checks every active job's state for NaN or infinity, takes the energy
+//      baseline of the jobs whose last bubble has been stamped (the first
+//      report after the last nucleation, headless.rs:54-78), and hands the
+//      jobs that reached their step count, or went non-finite, to the retire
+//      thread.
+// The retire thread owns a second stream.  For each handed-over job it waits
+// for the job's last step (an event on the stepping stream), runs the final
+// diagnostics, reads back the state when the job asked for it (checksum,
+// keep_state), returns the device memory and publishes the result, while
+// the worker keeps the other jobs stepping.
 // A job's arithmetic depends only on its own spec: tasks never span boxes and
-// the tile shape and z chunk are fixed for the server, so a job gives the
+// the tile shapes and z chunks are fixed for the server, so a job gives the
 // same bits alone and in any mix.
 #include <chrono>
 #include <condition_variable>
@@ -30,8 +38,17 @@ namespace {
 struct Job {
     uint64_t id = 0;
     bs_job job{};
+    std::vector<unsigned char> init;  // initial field then fluid bytes, when given
     double submit_s = 0, admit_s = 0;
     std::unique_ptr<Entry> entry;
+    int status = BS_OK;
+    // energy baseline: reduced on the stepping stream into the job's own
+    // scratch when its last event has been stamped, read back at retirement
+    ReducePartial *d_partials = nullptr;
+    ReduceResult *d_base = nullptr;
+    bool base_taken = false;
+    uint64_t base_step = 0;
+    hipEvent_t last_step = nullptr;  // recorded after the job's last step
 };
 
 struct Done {
@@ -45,19 +62,26 @@ struct Done {
 struct bs_server {
     bs_server_config cfg{};
     uint64_t budget = 0;
-    hipStream_t stream = nullptr;
-    std::thread worker;
+    hipStream_t stream = nullptr;   // stepping (worker)
+    hipStream_t rstream = nullptr;  // retirement (retire thread)
+    std::thread worker, retirer;
     std::mutex mu;
-    std::condition_variable cv;
-    bool stop = false;
+    std::condition_variable cv;   // results published, queue changed
+    std::condition_variable rcv;  // retire queue changed
+    bool stop = false;          // worker: stop stepping
+    bool retire_stop = false;   // retire thread: stop once its queue is empty
+    uint32_t n_active = 0;      // jobs admitted and not yet handed over
     std::deque<std::unique_ptr<Job>> queue;
-    std::vector<std::unique_ptr<Job>> active;
+    std::vector<std::unique_ptr<Job>> active;   // worker only
+    std::deque<std::unique_ptr<Job>> retiring;  // worker -> retire thread
     std::map<uint64_t, Done> done;
     uint64_t next_id = 1;
     bs_server_stats stats{};
     std::chrono::steady_clock::time_point t0;
     Engine eng;
-    ReduceScratch rs;
+    ReduceScratch rs;  // retire thread
+    uint32_t *h_flags = nullptr;  // pinned copy of the engine's non-finite flags
+    size_t h_flags_cap = 0;
     bool set_changed = true;
     int worker_error = 0;
     std::string worker_msg;
@@ -72,22 +96,44 @@ static uint64_t job_bytes(const bs_job &j) {
            j.spec.grid[2];
 }
 
-static void finish_job(bs_server *s, Job &j, int status) {
+// ---- retire thread ----------------------------------------------------------
+
+static void release_job(Job &j, hipStream_t st) {
+    if (j.entry) j.entry->release(st);
+    if (j.d_partials) (void)hipFreeAsync(j.d_partials, st);
+    if (j.d_base) (void)hipFreeAsync(j.d_base, st);
+    j.d_partials = nullptr;
+    j.d_base = nullptr;
+    if (j.last_step) (void)hipEventDestroy(j.last_step);
+    j.last_step = nullptr;
+}
+
+static void retire_one(bs_server *s, Job &j) {
     Done d;
     d.res.job_id = j.id;
-    d.res.status = status;
+    d.res.status = j.status;
     d.res.submit_s = j.submit_s;
     d.res.admit_s = j.admit_s;
-    if (j.entry && status == BS_OK) {
-        Entry *e = j.entry.get();
+    Entry *e = j.entry.get();
+    if (e && j.last_step) (void)hipStreamWaitEvent(s->rstream, j.last_step, 0);
+    if (e && (j.status == BS_OK || j.status == BS_ERR_NONFINITE)) {
         d.res.steps = e->steps_done;
-        if (launch_diagnostics(e->bx, s->rs, s->stream) == BS_OK && hipStreamSynchronize(s->stream) == hipSuccess)
+        bool ok = launch_diagnostics(e->bx, s->rs, s->rstream) == BS_OK;
+        ReduceResult base{};
+        if (ok && j.base_taken)
+            ok = hipMemcpyAsync(&base, j.d_base, sizeof base, hipMemcpyDeviceToHost, s->rstream) == hipSuccess;
+        if (ok && hipStreamSynchronize(s->rstream) == hipSuccess) {
             diagnostics_from_result(*s->rs.hres, e->bx.n_cells, &d.res.final_diag);
+            if (j.base_taken) {
+                diagnostics_from_result(base, e->bx.n_cells, &d.res.baseline_diag);
+                d.res.baseline_step = j.base_step;
+            }
+        }
         const size_t r = real_size(e->spec.precision);
         std::vector<unsigned char> buf;
-        if (j.job.keep_state || j.job.checksum) buf.resize(e->bx.n_cells * 6 * r);
+        if (j.status == BS_OK && (j.job.keep_state || j.job.checksum)) buf.resize(e->bx.n_cells * 6 * r);
         if (!buf.empty() &&
-            download_state(e->bx, s->stream, buf.data(), buf.data() + e->bx.n_cells * 2 * r) == BS_OK) {
+            download_state(e->bx, s->rstream, buf.data(), buf.data() + e->bx.n_cells * 2 * r) == BS_OK) {
             uint64_t h = 1469598103934665603ULL;
             for (unsigned char c : buf) {
                 h ^= c;
@@ -100,21 +146,148 @@ static void finish_job(bs_server *s, Job &j, int status) {
             }
         }
     }
-    if (j.entry) j.entry->release(s->stream);
+    release_job(j, s->rstream);
     d.res.finish_s = s->now();
     d.res.latency_s = d.res.finish_s - d.res.submit_s;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (status == BS_OK) {
+    if (j.status == BS_OK) {
         s->stats.completed += 1;
-        if (j.entry) s->stats.site_updates += j.entry->bx.n_cells * j.entry->steps_done;
+        if (e) s->stats.site_updates += e->bx.n_cells * e->steps_done;
+    } else {
+        s->stats.failed += 1;
+        if (j.status == BS_ERR_NONFINITE) s->stats.nonfinite += 1;
     }
     s->stats.bytes_in_use -= std::min<uint64_t>(s->stats.bytes_in_use, job_bytes(j.job));
     s->done[j.id] = std::move(d);
     s->cv.notify_all();
 }
 
+static void retire_loop(bs_server *s) {
+    (void)hipSetDevice(0);
+    while (true) {
+        std::unique_ptr<Job> j;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->rcv.wait(lk, [&] { return !s->retiring.empty() || s->retire_stop; });
+            if (s->retiring.empty()) break;  // stopping, nothing left
+            j = std::move(s->retiring.front());
+            s->retiring.pop_front();
+        }
+        retire_one(s, *j);
+    }
+    (void)hipStreamSynchronize(s->rstream);
+}
+
+// ---- worker -----------------------------------------------------------------
+
+// hand a job over to the retire thread; it is no longer stepped
+static void hand_over(bs_server *s, std::unique_ptr<Job> j, int status, bool was_active) {
+    if (j->status == BS_OK) j->status = status;
+    if (j->entry && hipEventCreateWithFlags(&j->last_step, hipEventDisableTiming) == hipSuccess) {
+        if (hipEventRecord(j->last_step, s->stream) != hipSuccess) {
+            (void)hipEventDestroy(j->last_step);
+            j->last_step = nullptr;
+            (void)hipStreamSynchronize(s->stream);
+        }
+    } else if (j->entry) {
+        j->last_step = nullptr;
+        (void)hipStreamSynchronize(s->stream);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (was_active) s->n_active -= 1;
+    s->retiring.push_back(std::move(j));
+    s->rcv.notify_all();
+}
+
+static int admit(bs_server *s, Job &j) {
+    j.entry.reset(new Entry());
+    int rc = j.entry->create(j.job.spec, s->stream, true);
+    if (!rc) {
+        hipError_t e = hipMallocAsync((void **)&j.d_partials, sizeof(ReducePartial) * kReduceBlocks, s->stream);
+        if (e == hipSuccess) e = hipMallocAsync((void **)&j.d_base, sizeof(ReduceResult), s->stream);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            rc = set_error(e == hipErrorOutOfMemory ? BS_ERR_OOM : BS_ERR_DEVICE,
+                           std::string("job scratch: ") + hipGetErrorString(e));
+        }
+    }
+    if (!rc && !j.init.empty()) {
+        // over the initialised slot, before anything reads it; the host
+        // bytes stay alive with the job until the copy has run
+        BoxState &bx = j.entry->bx;
+        const size_t fb = bx.n_cells * 2 * real_size(bx.precision);
+        hipError_t e = hipMemcpyAsync(bx.field[bx.cur], j.init.data(), fb, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(bx.fluid[bx.cur], j.init.data() + fb, j.init.size() - fb, hipMemcpyHostToDevice,
+                               s->stream);
+        if (e != hipSuccess) rc = set_error(BS_ERR_DEVICE, std::string("initial state: ") + hipGetErrorString(e));
+    }
+    j.entry->steps_target = j.job.steps;
+    return rc;
+}
+
+// the energy baseline of a job: once its last event has been stamped
+static int take_baseline(bs_server *s, Job &j) {
+    Entry *e = j.entry.get();
+    if (j.base_taken || e->steps_done <= e->last_event_step()) return BS_OK;
+    if (e->bx.precision == BS_PRECISION_F64) {
+        const Box<double> b = e->bx.box<double>();
+        BS_HIP_TRY(launch_reduce<double>(b.field[b.cur], b.fluid[b.cur], b.P, j.d_partials, j.d_base, s->stream));
+    } else {
+        const Box<float> b = e->bx.box<float>();
+        BS_HIP_TRY(launch_reduce<float>(b.field[b.cur], b.fluid[b.cur], b.P, j.d_partials, j.d_base, s->stream));
+    }
+    j.base_taken = true;
+    j.base_step = e->steps_done;
+    return BS_OK;
+}
+
+static int ensure_flags(bs_server *s, size_t n) {
+    if (n <= s->h_flags_cap) return BS_OK;
+    if (s->h_flags) (void)hipHostFree(s->h_flags);
+    s->h_flags = nullptr;
+    s->h_flags_cap = 0;
+    const size_t cap = std::max<size_t>(n, 64);
+    BS_HIP_TRY(hipHostMalloc((void **)&s->h_flags, cap * sizeof(uint32_t), hipHostMallocDefault));
+    s->h_flags_cap = cap;
+    return BS_OK;
+}
+
+// one tick over the active set: advance, non-finite check, baselines;
+// fills `bad` with the jobs found non-finite
+static int tick(bs_server *s, uint32_t tick_id, std::vector<char> &bad) {
+    int rc = BS_OK;
+    if (s->set_changed) {
+        s->eng.entries.clear();
+        for (auto &j : s->active) s->eng.entries.push_back(j->entry.get());
+        rc = s->eng.upload();
+        if (rc) return rc;
+    }
+    const bool reuse = !s->set_changed;
+    s->set_changed = false;
+    rc = s->eng.run(s->cfg.chunk_steps, reuse, (int)s->cfg.chunk_steps);
+    if (rc) return rc;
+    for (auto &j : s->active) j->entry->host_advance(s->cfg.chunk_steps);
+    rc = s->eng.check_finite(tick_id);
+    if (rc) return rc;
+    const size_t n = s->active.size();
+    rc = ensure_flags(s, n);
+    if (rc) return rc;
+    BS_HIP_TRY(hipMemcpyAsync(s->h_flags, s->eng.d_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    for (auto &j : s->active) {
+        rc = take_baseline(s, *j);
+        if (rc) return rc;
+    }
+    BS_HIP_TRY(hipStreamSynchronize(s->stream));
+    bad.assign(n, 0);
+    for (size_t i = 0; i < n; ++i) bad[i] = s->h_flags[i] == tick_id;
+    return BS_OK;
+}
+
 static void worker_loop(bs_server *s) {
     (void)hipSetDevice(0);
+    uint32_t tick_id = 0;
+    std::vector<char> bad;
     while (true) {
         // ---- admission ----
         std::vector<std::unique_ptr<Job>> admitted;
@@ -130,66 +303,70 @@ static void worker_loop(bs_server *s) {
                 admitted.push_back(std::move(s->queue.front()));
                 s->queue.pop_front();
             }
-        }
-        for (auto &j : admitted) {
-            j->entry.reset(new Entry());
-            int rc = j->entry->create(j->job.spec, s->stream, true);
-            j->admit_s = s->now();
-            if (rc) {
-                finish_job(s, *j, rc);
+            // nothing to step and nothing admissible: wait for retirements
+            // to return memory, or for new work
+            if (s->active.empty() && admitted.empty()) {
+                s->cv.wait_for(lk, std::chrono::milliseconds(50));
                 continue;
             }
-            j->entry->steps_target = j->job.steps;
+        }
+        for (auto &j : admitted) {
+            const int rc = admit(s, *j);
+            j->admit_s = s->now();
+            if (rc) {
+                hand_over(s, std::move(j), rc, false);
+                continue;
+            }
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->stats.admitted += 1;
-                s->active.push_back(std::move(j));
+                s->n_active += 1;
             }
+            s->active.push_back(std::move(j));
             s->set_changed = true;
         }
         if (s->active.empty()) continue;
 
         // ---- advance the active set ----
         const double tb = s->now();
-        int rc = BS_OK;
-        if (s->set_changed) {
-            s->eng.entries.clear();
-            for (auto &j : s->active) s->eng.entries.push_back(j->entry.get());
-            rc = s->eng.upload();
-        }
-        const bool reuse = !s->set_changed;
-        s->set_changed = false;
-        if (!rc) rc = s->eng.run(s->cfg.chunk_steps, reuse, (int)s->cfg.chunk_steps);
-        if (!rc) rc = s->eng.download_progress();
-        if (rc) {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->worker_error = rc;
-            s->worker_msg = bs_last_error();
-        }
+        const int rc = tick(s, ++tick_id, bad);
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->stats.ticks += 1;
             s->stats.busy_s += s->now() - tb;
-        }
-        // ---- retire ----
-        for (size_t i = 0; i < s->active.size();) {
-            Job &j = *s->active[i];
-            if (rc || j.entry->steps_done >= j.entry->steps_target) {
-                finish_job(s, j, rc ? rc : BS_OK);
-                {
-                    std::lock_guard<std::mutex> lk(s->mu);
-                    s->active.erase(s->active.begin() + (long)i);
-                }
-                s->set_changed = true;
+            if (rc) {
+                s->worker_error = rc;
+                s->worker_msg = bs_last_error();
             } else {
-                ++i;
+                s->worker_error = 0;  // a tick went through: the device works again
             }
         }
+        // ---- retire ----
+        std::vector<std::unique_ptr<Job>> keep;
+        for (size_t i = 0; i < s->active.size(); ++i) {
+            std::unique_ptr<Job> &j = s->active[i];
+            int status = BS_OK;
+            bool go = false;
+            if (rc) {
+                status = rc;  // a failed launch leaves no job's state trustworthy
+                go = true;
+            } else if (i < bad.size() && bad[i]) {
+                status = BS_ERR_NONFINITE;  // evicted early, as the headless run aborts (headless.rs:63-67)
+                go = true;
+            } else if (j->entry->steps_done >= j->entry->steps_target) {
+                go = true;
+            }
+            if (go) {
+                hand_over(s, std::move(j), status, true);
+                s->set_changed = true;
+            } else {
+                keep.push_back(std::move(j));
+            }
+        }
+        s->active.swap(keep);
     }
-    // shutdown: drop whatever is left
-    std::lock_guard<std::mutex> lk(s->mu);
-    for (auto &j : s->active)
-        if (j->entry) j->entry->release(s->stream);
+    // shutdown: hand what is left to the retire thread as cancelled
+    for (auto &j : s->active) hand_over(s, std::move(j), BS_ERR_NOT_READY, true);
     s->active.clear();
     (void)hipStreamSynchronize(s->stream);
 }
@@ -202,6 +379,11 @@ int bs_server_create(const bs_server_config *cfg, bs_server **out) {
     if (cfg->precision != BS_PRECISION_F32 && cfg->precision != BS_PRECISION_F64)
         return set_error(BS_ERR_INVALID, "invalid precision");
     if (bs_device_count() == 0) return set_error(BS_ERR_DEVICE, "no HIP device available");
+    const int math = cfg->precision == BS_PRECISION_F64 ? BS_MATH_EXACT : cfg->math;
+    TileShape t0, t1;
+    int rc = tile_from_env(TileShape{16, 16}, math, cfg->precision, &t0);
+    if (!rc) rc = tile_from_env(TileShape{32, 8}, math, cfg->precision, &t1);
+    if (rc) return rc;
     bs_server *s = new bs_server();
     s->cfg = *cfg;
     if (s->cfg.chunk_steps == 0) s->cfg.chunk_steps = 10;
@@ -213,24 +395,32 @@ int bs_server_create(const bs_server_config *cfg, bs_server **out) {
     s->budget = cfg->hbm_budget_bytes ? cfg->hbm_budget_bytes : (tot > cfg->reserve_bytes ? tot - cfg->reserve_bytes : 0);
     if (s->budget > fr) s->budget = fr > cfg->reserve_bytes ? fr - cfg->reserve_bytes : 0;
     s->stats.budget_bytes = s->budget;
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking) != hipSuccess) {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
         return set_error(BS_ERR_DEVICE, "hipStreamCreate failed");
     }
     s->eng.precision = cfg->precision;
     s->eng.math = cfg->math;
     s->eng.stream = s->stream;
-    s->eng.tile = tile_from_env(TileShape{16, 16});
-    s->eng.zc = 32;
-    int rc = s->rs.alloc();
+    s->eng.use_graphs = (cfg->flags & BS_SERVER_NO_GRAPHS) == 0;
+    // the first RK stage, bound by instruction issue, on 16x16 tiles; the
+    // other two, which also stream U^n, on 32x8 (as the single box)
+    s->eng.plan[0].tile = t0;
+    s->eng.plan[1].tile = t1;
+    s->eng.plan[0].zc = s->eng.plan[1].zc = 32;
+    rc = s->rs.alloc();
     if (rc) {
         const std::string msg = bs_last_error();
         (void)hipStreamDestroy(s->stream);
+        (void)hipStreamDestroy(s->rstream);
         delete s;
         return set_error(rc, msg);
     }
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread(worker_loop, s);
+    s->retirer = std::thread(retire_loop, s);
     *out = s;
     return BS_OK;
 }
@@ -252,9 +442,22 @@ int bs_server_submit(bs_server *s, const bs_job *job, uint64_t *job_id_out) {
         s->stats.rejected += 1;
         return set_error(rc, err);
     }
+    if ((job->init_field == nullptr) != (job->init_fluid == nullptr)) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stats.rejected += 1;
+        return set_error(BS_ERR_INVALID, "an initial state needs both the field and the fluid");
+    }
     std::unique_ptr<Job> j(new Job());
     j->job = *job;
     j->job.spec = sp;
+    if (job->init_field) {
+        const size_t cells = (size_t)sp.grid[0] * sp.grid[1] * sp.grid[2];
+        const size_t r = real_size(sp.precision);
+        j->init.resize(cells * 6 * r);
+        std::memcpy(j->init.data(), job->init_field, cells * 2 * r);
+        std::memcpy(j->init.data() + cells * 2 * r, job->init_fluid, cells * 4 * r);
+    }
+    j->job.init_field = j->job.init_fluid = nullptr;  // the copy above is the job's
     if (job_bytes(j->job) > s->budget) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.rejected += 1;
@@ -311,10 +514,15 @@ int bs_server_stats_get(bs_server *s, bs_server_stats *out) {
     if (!s || !out) return set_error(BS_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->stats;
-    out->active = (uint32_t)s->active.size();
+    out->active = s->n_active;
     out->queued = (uint32_t)s->queue.size();
     out->elapsed_s = s->now();
-    if (s->worker_error) return set_error(s->worker_error, s->worker_msg);
+    if (s->worker_error) {
+        // reported once; a later tick that goes through clears it as well
+        const int rc = s->worker_error;
+        s->worker_error = 0;
+        return set_error(rc, s->worker_msg);
+    }
     return BS_OK;
 }
 
@@ -326,10 +534,21 @@ void bs_server_destroy(bs_server *s) {
         s->cv.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->retire_stop = true;
+        s->rcv.notify_all();
+    }
+    if (s->retirer.joinable()) s->retirer.join();
     s->eng.entries.clear();
     s->rs.release();
+    if (s->h_flags) (void)hipHostFree(s->h_flags);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->eng.drop_graph();
+    s->eng.reap_graphs(true);
+    s->eng.stream = nullptr;  // destroyed below, before the engine
     if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->rstream) (void)hipStreamDestroy(s->rstream);
     delete s;
 }
 

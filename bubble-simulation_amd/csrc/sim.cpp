@@ -35,7 +35,22 @@ struct bs_sim {
 namespace {
 
 int sim_alloc(bs_sim *s) {
-    int rc = s->bx.alloc(s->spec.precision, s->spec.grid, s->stream, false);
+    // tile shapes first: a BS_TILE the arithmetic has no kernel for is
+    // rejected before anything is allocated
+    const int math = s->spec.precision == BS_PRECISION_F64 ? BS_MATH_EXACT : s->spec.math;
+    int rc = tile_from_env(TileShape{32, 8}, math, s->spec.precision, &s->tile);
+    if (rc) return rc;
+    s->zc = default_zc(s->spec.grid, s->tile);
+    // The first stage reads half the bytes of the other two and is bound by
+    // instruction issue, where 16x16 tiles (fewer halo and far-edge faces per
+    // cell) run 5 % faster at 512^3; stages 2 and 3 are bound by HBM, where
+    // 32x8 fetches less.  An explicit BS_TILE applies to every stage.
+    const bool fast_bag = s->spec.math == BS_MATH_FAST && s->spec.precision == BS_PRECISION_F32 &&
+                          s->spec.eos == BS_EOS_BAG && s->spec.kernel == BS_KERNEL_FUSED;
+    rc = tile_from_env(fast_bag ? TileShape{16, 16} : TileShape{32, 8}, math, s->spec.precision, &s->tile0);
+    if (rc) return rc;
+    s->zc0 = default_zc(s->spec.grid, s->tile0);
+    rc = s->bx.alloc(s->spec.precision, s->spec.grid, s->stream, false);
     if (rc) return rc;
     s->bx.set_eos(s->spec);
     if (s->spec.kernel == BS_KERNEL_TWO_PASS) {
@@ -47,16 +62,6 @@ int sim_alloc(bs_sim *s) {
             return set_error(BS_ERR_OOM, "out of device memory allocating the primitive cache");
         }
     }
-    s->tile = tile_from_env(TileShape{32, 8});
-    s->zc = default_zc(s->spec.grid, s->tile);
-    // The first stage reads half the bytes of the other two and is bound by
-    // instruction issue, where 16x16 tiles (fewer halo and far-edge faces per
-    // cell) run 5 % faster at 512^3; stages 2 and 3 are bound by HBM, where
-    // 32x8 fetches less.  An explicit BS_TILE applies to every stage.
-    const bool fast_bag = s->spec.math == BS_MATH_FAST && s->spec.precision == BS_PRECISION_F32 &&
-                          s->spec.eos == BS_EOS_BAG && s->spec.kernel == BS_KERNEL_FUSED;
-    s->tile0 = tile_from_env(fast_bag ? TileShape{16, 16} : TileShape{32, 8});
-    s->zc0 = default_zc(s->spec.grid, s->tile0);
     return BS_OK;
 }
 

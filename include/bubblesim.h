@@ -219,7 +219,8 @@ const char *bs_version(void);
 /* sizeof of each ABI struct, so bindings can check their mirrors
  * (parameter_blocks_have_the_expected_size, sim.rs:1070-1078).
  * which: 0 sim_params, 1 stage_params, 2 bubble, 3 nucleation_batch,
- * 4 diagnostics_gpu, 5 diagnostics, 6 spec, 7 model, 8 event, 9 derived. */
+ * 4 diagnostics_gpu, 5 diagnostics, 6 spec, 7 model, 8 event, 9 derived,
+ * 10 server_config, 11 job, 12 job_result, 13 server_stats. */
 size_t bs_struct_size(int which);
 
 /* ------------------------------------------------------------------------ */
@@ -409,13 +410,30 @@ int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid);
  * (e.g. non-finite) box to check that its neighbours are untouched. */
 int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fluid);
 int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out);
-/* Tile shape chosen by the autotuner for this batch (x, y). */
+/* Tile shape chosen by the autotuner for RK stages 2 and 3 of this batch
+ * (x, y). */
 int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty);
+/* The autotuner's choice for both classes of stage launch:
+ * out = (tx, ty, zc) of RK stage 1, then (tx, ty, zc) of stages 2 and 3.
+ * It times every built tile shape with z chunks of 16, 32 and 64 planes on
+ * the batch's boxes with their bubbles stamped, before the first step. */
+int bs_batch_plan(const bs_batch *b, uint32_t out[6]);
+/* Steps advanced so far through replay of a captured hipGraph (25 steps per
+ * graph; bs_batch_advance steps a remainder below that eagerly). */
+uint64_t bs_batch_graph_steps(const bs_batch *b);
 
 /* ------------------------------------------------------------------------ */
 /*  Serving: continuous batching of sweep jobs under an HBM budget          */
 /* ------------------------------------------------------------------------ */
 typedef struct bs_server bs_server;
+
+/* bs_server_config.flags */
+enum {
+    /* step every tick eagerly instead of replaying a captured hipGraph when
+     * the active set is unchanged (the results are the same bits); for
+     * tools that do not follow graph launches made from a second thread */
+    BS_SERVER_NO_GRAPHS = 1
+};
 
 typedef struct bs_server_config {
     uint64_t hbm_budget_bytes;   /* 0: use (total device memory - reserve)     */
@@ -424,6 +442,8 @@ typedef struct bs_server_config {
     uint32_t chunk_steps;        /* steps per scheduling tick (>= 1)           */
     int32_t precision;           /* BS_PRECISION_* for every job               */
     int32_t math;                /* BS_MATH_*                                  */
+    uint32_t flags;              /* BS_SERVER_*                                */
+    uint32_t pad;
 } bs_server_config;
 
 typedef struct bs_job {
@@ -431,19 +451,36 @@ typedef struct bs_job {
     uint64_t steps;              /* steps to advance                            */
     int32_t keep_state;          /* 1: keep the final state for bs_server_fetch */
     uint32_t checksum;           /* 1: fill state_checksum (reads the state back) */
+    /* Optional initial U^n (both or neither; layout as bs_sim_upload, in the
+     * server's precision), copied at submission and loaded over the
+     * initialised lattice when the job is admitted: a job restarted from a
+     * saved state.  Its schedule still runs from step 0.  NULL: the
+     * reference's initial state (init.wgsl:18-26). */
+    const void *init_field;
+    const void *init_fluid;
 } bs_job;
 
 typedef struct bs_job_result {
     uint64_t job_id;
     uint64_t steps;
     bs_diagnostics final_diag;
+    /* diagnostics at the first tick boundary after the job's last bubble was
+     * stamped, the headless run's energy baseline E0 (headless.rs:54-78):
+     * energy drift = final_diag.mean_energy / baseline_diag.mean_energy - 1 */
+    bs_diagnostics baseline_diag;
+    uint64_t baseline_step;      /* steps done when the baseline was taken     */
     double submit_s;             /* seconds since server start                 */
     double admit_s;
     double finish_s;
     double latency_s;            /* finish - submit                            */
     uint64_t state_checksum;     /* FNV-1a of the final field+fluid bytes, when
                                     requested (keep_state or checksum), else 0  */
-    int32_t status;              /* BS_OK or an error code                      */
+    int32_t status;              /* BS_OK; BS_ERR_NONFINITE when the state went
+                                    NaN/infinite (checked every tick; the job is
+                                    evicted at that tick, as the headless run
+                                    aborts, headless.rs:63-67); BS_ERR_OOM /
+                                    BS_ERR_DEVICE when admission or a launch
+                                    failed                                      */
     uint32_t pad;
 } bs_job_result;
 
@@ -456,8 +493,14 @@ typedef struct bs_server_stats {
     uint32_t active, queued;
     double busy_s;               /* wall time spent advancing                  */
     double elapsed_s;            /* since start                                */
+    uint64_t failed;             /* jobs finished with a status other than BS_OK */
+    uint64_t nonfinite;          /* of which evicted as non-finite             */
 } bs_server_stats;
 
+/* Starts two host threads: a worker that admits jobs and steps the active
+ * set on one stream, and a retire thread that, on a second stream, reads
+ * back each finished job's diagnostics (and state, when asked for) and
+ * returns its memory while the worker keeps the other jobs stepping. */
 int bs_server_create(const bs_server_config *cfg, bs_server **out);
 /* Thread-safe.  Rejects jobs that can never fit the budget (BS_ERR_OOM) and
  * invalid specs (BS_ERR_INVALID). */
@@ -468,6 +511,9 @@ int bs_server_poll(bs_server *s, uint64_t job_id, bs_job_result *out);
 int bs_server_wait(bs_server *s, uint64_t job_id, double timeout_s, bs_job_result *out);
 /* Final state of a job submitted with keep_state (layout as bs_sim_download). */
 int bs_server_fetch_state(bs_server *s, uint64_t job_id, void *field, void *fluid);
+/* Counters so far.  Returns the error of the most recent tick when that
+ * tick failed (every job active in it then finished with that error), once;
+ * a later tick that goes through clears it as well. */
 int bs_server_stats_get(bs_server *s, bs_server_stats *out);
 /* Stop accepting work, finish nothing further, free device memory. */
 void bs_server_destroy(bs_server *s);

@@ -47,6 +47,10 @@ def test_parameter_blocks_have_the_expected_size(bs):
     assert L.bs_struct_size(7) == C.sizeof(bs.CModel)
     assert L.bs_struct_size(8) == C.sizeof(bs.CEvent) == 20
     assert L.bs_struct_size(9) == C.sizeof(bs.CDerived)
+    assert L.bs_struct_size(10) == C.sizeof(bs.CServerConfig) == 40
+    assert L.bs_struct_size(11) == C.sizeof(bs.CJob)
+    assert L.bs_struct_size(12) == C.sizeof(bs.CJobResult)
+    assert L.bs_struct_size(13) == C.sizeof(bs.CServerStats)
 
 
 def test_stage_slots_never_alias_output_with_input(bs):

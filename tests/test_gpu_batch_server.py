@@ -31,17 +31,21 @@ def _specs(bs, math, n=6, seed=3):
 
 
 def test_batch_equals_single_boxes_exact(bs, gpu_available):
+    # 6 eager steps (every event due by step 8 is stamped in them or in the
+    # graph), then 30 steps: one 25-step graph replayed, 5 eager
     specs = _specs(bs, bs.MATH_EXACT)
     b = bs.Batch(specs)
     b.advance(6, use_graph=False)
-    b.advance(9, use_graph=True)
+    assert b.graph_steps() == 0
+    b.advance(30, use_graph=True)
+    assert b.graph_steps() == 25
     for i, s in enumerate(specs):
         print(f"box {i} grid {tuple(s.grid)}", flush=True)
         sim = bs.Simulation(s)
-        sim.advance(15)
+        sim.advance(36)
         f, u = sim.download()
         bf, bu = b.download(i)
-        assert b.steps(i) == 15
+        assert b.steps(i) == 36
         assert np.array_equal(f, bf) and np.array_equal(u, bu)
         d1, d2 = sim.diagnostics(), b.diagnostics(i)
         assert d1.mean_energy == d2.mean_energy and d1.max_velocity == d2.max_velocity
@@ -53,8 +57,10 @@ def test_batch_graph_equals_eager_fast(bs, gpu_available):
     specs = _specs(bs, bs.MATH_FAST, n=5, seed=9)
     a = bs.Batch(specs)
     a.advance(60, use_graph=True)
+    assert a.graph_steps() == 50
     c = bs.Batch(specs)
     c.advance(60, use_graph=False)
+    assert c.graph_steps() == 0
     for i in range(len(specs)):
         fa, ua = a.download(i)
         fc, uc = c.download(i)
