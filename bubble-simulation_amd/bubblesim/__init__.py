@@ -159,6 +159,7 @@ _SIGS = {
     "bs_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "bs_device_name": (C.c_int, [C.c_char_p, C.c_size_t]),
     "bs_hbm_probe": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(CHbmRates)]),
+    "bs_reduce_noalloc": (C.c_int, [C.POINTER(CSimParams), C.c_int32, _P, _P, _P, _P]),
     "bs_sim_create": (C.c_int, [C.POINTER(CSpec), C.POINTER(_P)]),
     "bs_sim_destroy": (None, [_P]),
     "bs_sim_reset": (C.c_int, [_P, C.POINTER(CEvent), C.c_uint32]),

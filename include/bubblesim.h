@@ -387,6 +387,34 @@ int bs_sim_upload(bs_sim *sim, const void *field, const void *fluid);
  * around each launch, and the wall time of the whole call. */
 int bs_sim_advance_timed(bs_sim *sim, uint64_t n, double *stage_kernel_ms, double *wall_ms);
 
+/* Result of bs_reduce_noalloc, written to device memory: the FP64
+ * compensated sums of reduce.wgsl's per-cell terms (e_total, broken,
+ * kinetic, E), max |v|, and a non-zero `nonfinite` if any term was NaN/Inf.
+ * bs_diagnostics' means are these sums divided by the cell count. */
+typedef struct bs_reduce_result {
+    double sums[4];
+    double max_velocity;
+    double nonfinite;
+} bs_reduce_result;
+
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under hipGraph capture, tight serving loops), the diagnostics reduction
+ * of bs_sim_diagnostics over a lattice the caller holds in device memory,
+ * allocating nothing.  field/fluid: device pointers to one state (layout as
+ * bs_sim_download, in `precision`; the bag equation of state); params: its
+ * SimParams block (bs_sim_params_fill); out_device: device memory for one
+ * bs_reduce_result; stream: the hipStream_t to enqueue on (NULL: the null
+ * stream).  Enqueues the reduction and returns.  The reduction's 512
+ * partial results live in a fixed 40 KB device array of the library, one
+ * per device, held until the process ends, whatever the lattice's size;
+ * lattices bs_spec_validate would reject (an axis below 8 cells, an x-y
+ * plane of 2^28 cells or more in f32, 2^27 in f64) are rejected with
+ * BS_ERR_INVALID.  Calls that overlap on one device share that array and
+ * race: serialise them (one stream, or events).  The sums equal
+ * bs_sim_diagnostics' bit for bit on the same state. */
+int bs_reduce_noalloc(const bs_sim_params *params, int32_t precision, const void *field, const void *fluid,
+                      void *out_device, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /*  Batched sweep: many independent boxes advanced together (no reference   */
 /*  counterpart; each box follows Simulation semantics exactly)            */
