@@ -201,6 +201,7 @@ _SIGS = {
     "bs_batch_tile": (C.c_int, [_P, _u32p, _u32p]),
     "bs_batch_plan": (C.c_int, [_P, _u32p]),
     "bs_batch_graph_steps": (C.c_uint64, [_P]),
+    "bs_batch_streams": (C.c_uint32, [_P]),
     "bs_server_create": (C.c_int, [C.POINTER(CServerConfig), C.POINTER(_P)]),
     "bs_server_submit": (C.c_int, [_P, C.POINTER(CJob), C.POINTER(C.c_uint64)]),
     "bs_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(CJobResult)]),
@@ -656,6 +657,10 @@ class Batch:
     def graph_steps(self) -> int:
         """Steps advanced so far through replay of a captured hipGraph."""
         return int(lib().bs_batch_graph_steps(self._h))
+
+    def streams(self) -> int:
+        """Stream groups the boxes are spread over."""
+        return int(lib().bs_batch_streams(self._h))
 
     def download(self, box: int):
         s = self.specs[box]

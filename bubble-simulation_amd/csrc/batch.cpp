@@ -1,7 +1,16 @@
 // batch.cpp -- bs_batch: a parameter sweep of independent boxes advanced
-// together (one launch per RK stage across all boxes, steps captured into a
-// hipGraph).  Each box follows the single-box semantics exactly: events are
+// together.  Each box follows the single-box semantics exactly: events are
 // stamped before the step whose f32 clock first reaches their time.
+//
+// The boxes are split into groups of about equal cell count, each advanced
+// by its own engine on its own stream: one fused launch per RK stage across
+// the group's boxes, steps captured into a hipGraph.  The groups' launches
+// run concurrently, so while one group's stage launch drains its last
+// workgroups, the other groups' work fills the compute units.  A box's
+// arithmetic does not depend on its group (tasks never span boxes, and every
+// group uses the one tile plan the autotuner chose for the whole batch).
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "multibox.h"
@@ -11,23 +20,44 @@ using namespace bs;
 // steps captured per graph (bs_batch_advance replays it, and steps the
 // remainder eagerly)
 constexpr int kBatchGraphSteps = 25;
+// stream groups of a batch (BS_BATCH_STREAMS overrides, 1..4: the device
+// keeps 4 hardware queues per process)
+constexpr int kBatchGroups = 2;
+
+namespace {
+struct Group {
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Engine> eng;
+};
+}  // namespace
 
 struct bs_batch {
     int precision = BS_PRECISION_F32;
     int math = BS_MATH_FAST;
-    hipStream_t stream = nullptr;
     std::vector<std::unique_ptr<Entry>> entries;
-    Engine eng;
+    std::vector<Group> groups;
+    std::vector<int> group_of;  // box -> group
     ReduceScratch rs;
     bool progress_valid = true;  // host mirrors match the device
 };
 
 static int batch_sync_progress(bs_batch *b) {
     if (b->progress_valid) return BS_OK;
-    int rc = b->eng.download_progress();
-    if (rc) return rc;
+    for (Group &g : b->groups) {
+        int rc = g.eng->download_progress();
+        if (rc) return rc;
+    }
     b->progress_valid = true;
     return BS_OK;
+}
+
+static hipStream_t box_stream(const bs_batch *b, uint32_t box) { return b->groups[b->group_of[box]].stream; }
+
+static int group_count(uint32_t n) {
+    int k = kBatchGroups;
+    if (const char *e = std::getenv("BS_BATCH_STREAMS")) k = std::atoi(e);
+    k = std::max(1, std::min(k, 4));
+    return (int)std::min<uint32_t>((uint32_t)k, n);
 }
 
 extern "C" {
@@ -43,28 +73,70 @@ int bs_batch_create(const bs_spec *specs, uint32_t n, bs_batch **out) {
     bs_batch *b = new bs_batch();
     b->precision = specs[0].precision;
     b->math = specs[0].math;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
-        return set_error(BS_ERR_DEVICE, "hipStreamCreate failed");
+    const int ng = group_count(n);
+    for (int g = 0; g < ng; ++g) {
+        Group gr;
+        if (hipStreamCreateWithFlags(&gr.stream, hipStreamNonBlocking) != hipSuccess) {
+            bs_batch_destroy(b);
+            return set_error(BS_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        gr.eng.reset(new Engine());
+        gr.eng->precision = b->precision;
+        gr.eng->math = b->math;
+        gr.eng->stream = gr.stream;
+        b->groups.push_back(std::move(gr));
     }
-    b->eng.precision = b->precision;
-    b->eng.math = b->math;
-    b->eng.stream = b->stream;
+    hipStream_t s0 = b->groups[0].stream;
     int rc = b->rs.alloc();
     for (uint32_t i = 0; i < n && !rc; ++i) {
         std::unique_ptr<Entry> e(new Entry());
-        rc = e->create(specs[i], b->stream, false);
+        rc = e->create(specs[i], s0, false);
         if (rc) {
             const std::string msg = bs_last_error();
-            e->release(b->stream);
+            e->release(s0);
             rc = set_error(rc, "box " + std::to_string(i) + ": " + msg);
             break;
         }
         b->entries.push_back(std::move(e));
     }
     if (!rc) {
-        for (auto &e : b->entries) b->eng.entries.push_back(e.get());
-        rc = autotune(b->eng);
+        // one tile plan for the whole batch, timed over every box
+        Engine tune;
+        tune.precision = b->precision;
+        tune.math = b->math;
+        tune.stream = s0;
+        for (auto &e : b->entries) tune.entries.push_back(e.get());
+        rc = autotune(tune);
+        if (!rc) {
+            // boxes to groups, largest first onto the group with the fewest cells
+            std::vector<uint32_t> order(n);
+            for (uint32_t i = 0; i < n; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+                return b->entries[a]->bx.n_cells > b->entries[c]->bx.n_cells;
+            });
+            std::vector<uint64_t> load(ng, 0);
+            b->group_of.assign(n, 0);
+            for (uint32_t i : order) {
+                const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                b->group_of[i] = g;
+                load[g] += b->entries[i]->bx.n_cells;
+            }
+            for (uint32_t i = 0; i < n; ++i) b->groups[b->group_of[i]].eng->entries.push_back(b->entries[i].get());
+            // the entries were initialised on the first group's stream
+            if (hipStreamSynchronize(s0) != hipSuccess) rc = set_error(BS_ERR_DEVICE, "hipStreamSynchronize failed");
+            for (Group &g : b->groups) {
+                if (rc) break;
+                g.eng->plan[0].tile = tune.plan[0].tile;
+                g.eng->plan[0].zc = tune.plan[0].zc;
+                g.eng->plan[1].tile = tune.plan[1].tile;
+                g.eng->plan[1].zc = tune.plan[1].zc;
+                rc = g.eng->upload();
+            }
+            for (Group &g : b->groups)
+                if (!rc && hipStreamSynchronize(g.stream) != hipSuccess)
+                    rc = set_error(BS_ERR_DEVICE, "hipStreamSynchronize failed");
+        }
+        tune.entries.clear();
     }
     if (rc) {
         const std::string msg = bs_last_error();
@@ -77,15 +149,20 @@ int bs_batch_create(const bs_spec *specs, uint32_t n, bs_batch **out) {
 
 void bs_batch_destroy(bs_batch *b) {
     if (!b) return;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    b->eng.entries.clear();
-    for (auto &e : b->entries) e->release(b->stream);
+    for (Group &g : b->groups)
+        if (g.stream) (void)hipStreamSynchronize(g.stream);
+    hipStream_t s0 = b->groups.empty() ? nullptr : b->groups[0].stream;
+    for (Group &g : b->groups) g.eng->entries.clear();
+    for (auto &e : b->entries) e->release(s0);
     b->entries.clear();
     b->rs.release();
-    b->eng.drop_graph();
-    b->eng.reap_graphs(true);
-    b->eng.stream = nullptr;  // destroyed below, before the engine
-    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (Group &g : b->groups) {
+        g.eng->drop_graph();
+        g.eng->reap_graphs(true);
+        g.eng->stream = nullptr;  // destroyed below, before the engine
+        g.eng.reset();
+        if (g.stream) (void)hipStreamDestroy(g.stream);
+    }
     delete b;
 }
 
@@ -94,12 +171,17 @@ int bs_batch_advance(bs_batch *b, uint64_t n, int32_t use_graph) {
     int rc = batch_sync_progress(b);
     if (rc) return rc;
     b->progress_valid = false;
-    return b->eng.run(n, use_graph != 0, kBatchGraphSteps);
+    // every group's launches are enqueued before any is waited for
+    for (Group &g : b->groups) {
+        rc = g.eng->run(n, use_graph != 0, kBatchGraphSteps);
+        if (rc) return rc;
+    }
+    return BS_OK;
 }
 
 int bs_batch_synchronize(bs_batch *b) {
     if (!b) return set_error(BS_ERR_INVALID, "null batch");
-    BS_HIP_TRY(hipStreamSynchronize(b->stream));
+    for (Group &g : b->groups) BS_HIP_TRY(hipStreamSynchronize(g.stream));
     return BS_OK;
 }
 
@@ -118,9 +200,10 @@ int bs_batch_diagnostics(bs_batch *b, uint32_t box, bs_diagnostics *out) {
     int rc = batch_sync_progress(b);
     if (rc) return rc;
     Entry *e = b->entries[box].get();
-    rc = launch_diagnostics(e->bx, b->rs, b->stream);
+    hipStream_t st = box_stream(b, box);
+    rc = launch_diagnostics(e->bx, b->rs, st);
     if (rc) return rc;
-    BS_HIP_TRY(hipStreamSynchronize(b->stream));
+    BS_HIP_TRY(hipStreamSynchronize(st));
     diagnostics_from_result(*b->rs.hres, e->bx.n_cells, out);
     return BS_OK;
 }
@@ -130,7 +213,7 @@ int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid) {
     if (box >= b->entries.size()) return set_error(BS_ERR_NOT_FOUND, "no such box");
     int rc = batch_sync_progress(b);
     if (rc) return rc;
-    return download_state(b->entries[box]->bx, b->stream, field, fluid);
+    return download_state(b->entries[box]->bx, box_stream(b, box), field, fluid);
 }
 
 int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fluid) {
@@ -140,9 +223,10 @@ int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fl
     if (rc) return rc;
     BoxState &bx = b->entries[box]->bx;
     const size_t r = real_size(bx.precision);
-    BS_HIP_TRY(hipStreamSynchronize(b->stream));
-    if (field) BS_HIP_TRY(hipMemcpy(bx.field[bx.cur], field, bx.n_cells * 2 * r, hipMemcpyHostToDevice));
-    if (fluid) BS_HIP_TRY(hipMemcpy(bx.fluid[bx.cur], fluid, bx.n_cells * 4 * r, hipMemcpyHostToDevice));
+    hipStream_t st = box_stream(b, box);
+    if (field) BS_HIP_TRY(hipMemcpyAsync(bx.field[bx.cur], field, bx.n_cells * 2 * r, hipMemcpyHostToDevice, st));
+    if (fluid) BS_HIP_TRY(hipMemcpyAsync(bx.fluid[bx.cur], fluid, bx.n_cells * 4 * r, hipMemcpyHostToDevice, st));
+    BS_HIP_TRY(hipStreamSynchronize(st));
     return BS_OK;
 }
 
@@ -157,21 +241,30 @@ int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out) {
 
 int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty) {
     if (!b) return set_error(BS_ERR_INVALID, "null batch");
-    if (tx) *tx = (uint32_t)b->eng.plan[1].tile.tx;
-    if (ty) *ty = (uint32_t)b->eng.plan[1].tile.ty;
+    const Plan &p = b->groups[0].eng->plan[1];
+    if (tx) *tx = (uint32_t)p.tile.tx;
+    if (ty) *ty = (uint32_t)p.tile.ty;
     return BS_OK;
 }
 
 int bs_batch_plan(const bs_batch *b, uint32_t out[6]) {
     if (!b || !out) return set_error(BS_ERR_INVALID, "null argument");
     for (int c = 0; c < 2; ++c) {
-        out[3 * c + 0] = (uint32_t)b->eng.plan[c].tile.tx;
-        out[3 * c + 1] = (uint32_t)b->eng.plan[c].tile.ty;
-        out[3 * c + 2] = (uint32_t)b->eng.plan[c].zc;
+        const Plan &p = b->groups[0].eng->plan[c];
+        out[3 * c + 0] = (uint32_t)p.tile.tx;
+        out[3 * c + 1] = (uint32_t)p.tile.ty;
+        out[3 * c + 2] = (uint32_t)p.zc;
     }
     return BS_OK;
 }
 
-uint64_t bs_batch_graph_steps(const bs_batch *b) { return b ? b->eng.graph_replayed : 0; }
+uint64_t bs_batch_graph_steps(const bs_batch *b) {
+    if (!b || b->groups.empty()) return 0;
+    uint64_t m = UINT64_MAX;
+    for (const Group &g : b->groups) m = std::min(m, g.eng->graph_replayed);
+    return m;
+}
+
+uint32_t bs_batch_streams(const bs_batch *b) { return b ? (uint32_t)b->groups.size() : 0; }
 
 }  // extern "C"

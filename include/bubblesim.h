@@ -449,6 +449,10 @@ int bs_batch_plan(const bs_batch *b, uint32_t out[6]);
 /* Steps advanced so far through replay of a captured hipGraph (25 steps per
  * graph; bs_batch_advance steps a remainder below that eagerly). */
 uint64_t bs_batch_graph_steps(const bs_batch *b);
+/* Streams the batch's boxes are spread over: groups of about equal cell
+ * count, each stepped by its own launches (and graph) on its own stream, so
+ * that the groups' launches overlap on the device. */
+uint32_t bs_batch_streams(const bs_batch *b);
 
 /* ------------------------------------------------------------------------ */
 /*  Serving: continuous batching of sweep jobs under an HBM budget          */
