@@ -213,6 +213,14 @@ def f64_spec(bs, spec):
     return s64
 
 
+def exact32_spec(bs, spec):
+    """The same box in f32 ExactMath (the reference's operation order)."""
+    s = bs.CSpec.from_buffer_copy(spec)
+    s.precision = bs.PRECISION_F32
+    s.math = bs.MATH_EXACT
+    return s
+
+
 def accuracy_sample(specs):
     """Indices of the first box of each edge 32, 64, 96, 128 (or fewer)."""
     idx, seen = [], set()
@@ -254,20 +262,28 @@ def batched_sweep(bs, steps=1000, report_at=200):
     if any(e1[i].nonfinite for i in sample):
         raise RuntimeError("a batched box went non-finite")
     # f64 of the sample through the same engine
-    b64 = bs.Batch([f64_spec(bs, specs[i]) for i in sample])
-    b64.advance(report_at, use_graph=True)
-    f0 = [b64.diagnostics(k).mean_energy for k in range(len(sample))]
-    b64.advance(steps - report_at, use_graph=True)
-    f1 = [b64.diagnostics(k).mean_energy for k in range(len(sample))]
-    ref = [b64.download(k) for k in range(len(sample))]
-    b64.close()
+    def sample_run(make):
+        bb = bs.Batch([make(bs, specs[i]) for i in sample])
+        bb.advance(report_at, use_graph=True)
+        a0 = [bb.diagnostics(k).mean_energy for k in range(len(sample))]
+        bb.advance(steps - report_at, use_graph=True)
+        a1 = [bb.diagnostics(k).mean_energy for k in range(len(sample))]
+        st = [bb.download(k) for k in range(len(sample))]
+        bb.close()
+        return a0, a1, st
+
+    f0, f1, ref = sample_run(f64_spec)          # the accuracy reference
+    x0, x1, exact32 = sample_run(exact32_spec)  # the reference's f32 operation order, for scale
     acc = {"reference": "the same boxes in f64 ExactMath through the batched engine", "steps": steps,
            "boxes": [{"index": i, "edge": int(specs[i].grid[0]),
                       "energy_drift_f32": e1[i].mean_energy / e0[i] - 1.0,
+                      "energy_drift_f32_exact": x1[k] / x0[k] - 1.0,
                       "energy_drift_f64": f1[k] / f0[k] - 1.0} for k, i in enumerate(sample)],
-           "drift_from_step": report_at, "blocks": field_errors(states, ref)}
+           "drift_from_step": report_at, "blocks": field_errors(states, ref),
+           "blocks_f32_exact": field_errors(exact32, ref)}
     log("bench: batched accuracy (sample " + ", ".join(str(int(specs[i].grid[0])) for i in sample) + "): rel L2 "
-        + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()))
+        + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()) + "; f32 exact "
+        + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks_f32_exact"].items()))
     seq_t, seq_work = 0.0, 0
     for s in specs:
         sim = bs.Simulation(s)
@@ -331,14 +347,19 @@ def served_accuracy(bs, jobs, keep, kept, res):
     in f64 ExactMath through a server; the drift of each is taken from the
     job's baseline diagnostics (first tick after its last nucleation) to its
     final ones."""
-    srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20, precision=bs.PRECISION_F64,
-                    math=bs.MATH_EXACT)
-    try:
-        ids = [srv.submit(f64_spec(bs, jobs[k][1]), jobs[k][2], keep_state=True) for k in keep]
-        r64 = [srv.wait(j, 600) for j in ids]
-        ref = [srv.fetch_state(j, f64_spec(bs, jobs[k][1])) for j, k in zip(ids, keep)]
-    finally:
-        srv.close()
+    def sample_run(precision, make):
+        srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20, precision=precision,
+                        math=bs.MATH_EXACT)
+        try:
+            ids = [srv.submit(make(bs, jobs[k][1]), jobs[k][2], keep_state=True) for k in keep]
+            rr = [srv.wait(j, 600) for j in ids]
+            st = [srv.fetch_state(j, jobs[k][1]) for j, k in zip(ids, keep)]
+        finally:
+            srv.close()
+        return rr, st
+
+    r64, ref = sample_run(bs.PRECISION_F64, f64_spec)
+    r32, exact32 = sample_run(bs.PRECISION_F32, exact32_spec)
 
     def drift(r):
         return r.final_diag.mean_energy / r.baseline_diag.mean_energy - 1.0 if r.baseline_step else None
@@ -346,8 +367,9 @@ def served_accuracy(bs, jobs, keep, kept, res):
     return {"reference": "the same jobs in f64 ExactMath through the server",
             "jobs": [{"index": k, "edge": int(jobs[k][1].grid[0]), "steps": jobs[k][2],
                       "drift_from_step": int(res[k].baseline_step), "energy_drift_f32": drift(res[k]),
-                      "energy_drift_f64": drift(r)} for k, r in zip(keep, r64)],
-            "blocks": field_errors([kept[k] for k in keep], ref)}
+                      "energy_drift_f32_exact": drift(x), "energy_drift_f64": drift(r)}
+                     for k, r, x in zip(keep, r64, r32)],
+            "blocks": field_errors([kept[k] for k in keep], ref), "blocks_f32_exact": field_errors(exact32, ref)}
 
 
 def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
@@ -389,7 +411,8 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     acc = served_accuracy(bs, jobs, keep, kept, res) if len(kept) == len(keep) else None
     if acc:
         log("bench: served accuracy (sample edges " + ", ".join(str(j["edge"]) for j in acc["jobs"]) + "): rel L2 "
-            + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()))
+            + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()) + "; f32 exact "
+            + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks_f32_exact"].items()))
     return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "budget_bytes": int(st.budget_bytes),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,

@@ -1,6 +1,8 @@
 // kernels_fast.hip -- FastMathF32 instantiations of the stage kernels.
-// Built with -ffp-contract=fast: the compiler fuses multiply-adds.  Only the
-// tiles with one halo item per thread are built (tile_built in launch.h).
+// Built with -ffp-contract=fast-honor-pragmas: the compiler fuses
+// multiply-adds except in the RK stage combination (scheme.h, stage_update).
+// Only the tiles with one halo item per thread are built (tile_built in
+// launch.h).
 #include "launch.h"
 
 namespace bs {
