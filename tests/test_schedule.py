@@ -9,8 +9,9 @@ Pins, in order of strength:
 * the reference's own test (schedule_respects_ordering_and_bounds,
   physics.rs:387-401) and the rejection rule (2.5 R0 periodic separation).
 The f32 conversion `(next_u32 >> 8) * 2^-24` of rand 0.9's StandardUniform
-is pinned by its documented algorithm only (no published vector available
-here); the fixtures record it.
+is pinned by rand's own value-stability vector (three f32 draws from its
+PCG32 test generator; the f64 conversion beside it fixes the order in which
+PCG32 composes a u64 from two u32 words, so a shifted or swapped word fails).
 """
 import json
 import math
@@ -52,6 +53,55 @@ def test_f32_draws_are_24_bit_fractions(bs):
     expect = ((u & np.uint64(0xFFFFFFFF)) >> np.uint64(8)).astype(np.float64) * 2.0 ** -24
     assert np.array_equal(f.astype(np.float64), expect)
     assert f.min() >= 0.0 and f.max() < 1.0
+
+
+class _Pcg32:
+    """rand_pcg Lcg64Xsh32 (PCG-XSH-RR 64/32), the generator of rand's own
+    float value-stability test. Pinned by the PCG demo vector below."""
+    MUL = 6364136223846793005
+    M64 = (1 << 64) - 1
+
+    def __init__(self, state: int, stream: int):
+        self.inc = ((stream << 1) | 1) & self.M64
+        self.state = (state + self.inc) & self.M64
+        self._step()
+
+    def _step(self):
+        self.state = (self.state * self.MUL + self.inc) & self.M64
+
+    def next_u32(self) -> int:
+        s = self.state
+        self._step()
+        rot = s >> 59
+        xsh = (((s >> 18) ^ s) >> 27) & 0xFFFFFFFF
+        return ((xsh >> rot) | (xsh << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+    def next_u64(self) -> int:
+        lo = self.next_u32()
+        return (self.next_u32() << 32) | lo
+
+
+def test_f32_draw_matches_rand_value_stability_vector():
+    """rand 0.9 src/distr/float.rs value_stability: the oracle's f32 draw
+    (the method the schedule uses, driven by a PCG32 stream) reproduces
+    rand's published samples bit for bit."""
+    pub = json.load(open(os.path.join(GOLDEN, "rng_published.json")))["rand_float_value_stability"]
+    r = _Pcg32(42, 54)
+    assert [r.next_u32() for _ in range(6)] == [int(h, 16) for h in pub["pcg32_42_54"]]
+    state, stream = int(pub["pcg32_state"], 16), int(pub["pcg32_stream"])
+    r = _Pcg32(state, stream)
+    got = np.array([S.Mcg128Xsl64.random_f32(r) for _ in range(3)], dtype=np.float32)
+    want = np.array([np.float32(v) for v in pub["f32"]], dtype=np.float32)
+    assert got.view(np.uint32).tolist() == want.view(np.uint32).tolist()
+    r = _Pcg32(state, stream)
+    assert [(r.next_u64() >> 11) * 2.0 ** -53 for _ in range(3)] == [float(v) for v in pub["f64"]]
+
+
+def test_library_f32_draw_equals_oracle_draw(bs):
+    for seed in (0, 5, 0x5EED, 2**63 + 7):
+        r = S.Mcg128Xsl64.seed_from_u64(seed)
+        want = np.array([r.random_f32() for _ in range(64)], dtype=np.float32)
+        assert bs.pcg64mcg_f32(seed, 64).view(np.uint32).tolist() == want.view(np.uint32).tolist()
 
 
 def _lib_schedule(bs, c):
