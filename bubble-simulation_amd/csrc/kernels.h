@@ -317,6 +317,11 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
                                            int x0, int y0, int zb, int ze, int stage) {
     constexpr int NT = TX * TY;
     const int tid = threadIdx.x;
+    // halo items are numbered from the second wave on, so the first wave,
+    // which also evaluates the far-edge faces, carries no halo item (the
+    // waves of a block meet at two barriers per plane, so the heaviest
+    // wave sets the pace; measured 0.6 % faster at 512^3)
+    const int htid = NT % 64 == 0 && NT > 64 ? (tid >= 64 ? tid - 64 : tid - 64 + NT) : tid;
     const int tx = tid % TX;
     const int ty = tid / TX;
     const int gx = wrapi(x0 + tx, P.nx);
@@ -337,7 +342,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     int hphi[HPT];  // (hy + 1) * (TX + 2) + (hx + 1) for the 1-cell halo, or -1
 #pragma unroll
     for (int k = 0; k < HPT; ++k) {
-        const int h = tid + k * NT;
+        const int h = htid + k * NT;
         hlds[k] = -1;
         hphi[k] = -1;
         hoff[k] = 0;
@@ -434,7 +439,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                const int h = tid + k * NT;
+                const int h = htid + k * NT;
                 L.hq[slot][h] = prim_of(hfr[k], hur[k]);
                 L.hph[slot][h] = hfr[k].x;
             }
@@ -488,7 +493,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 #pragma unroll
         for (int k = 0; k < HPT; ++k) {
             if (hlds[k] >= 0) {
-                const int h = tid + k * NT;
+                const int h = htid + k * NT;
                 qflat[hlds[k]] = L.hq[I0][h];
                 if (hphi[k] >= 0) phiflat[hphi[k]] = L.hph[I0][h];
             }
@@ -499,8 +504,9 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
         if constexpr (XSHARE) {
             const V4<R> h = half_slope4<M>(sub4(qz, L.q[ty + 2][tx + 1]), sub4(L.q[ty + 2][tx + 3], qz));
             R v2r, v2l;
-            const V4<R> right = sanitise_v2<M>(add4(qz, h), P, v2r);  // left state of the face at x + 1/2
-            const V4<R> b = sanitise_v2<M>(sub4(qz, h), P, v2l);      // right state of the face at x - 1/2
+            V4<R> right = add4(qz, h);  // left state of the face at x + 1/2
+            V4<R> b = sub4(qz, h);      // right state of the face at x - 1/2
+            sanitise_pair_v2<M>(right, v2r, b, v2l, P);
             const V4<R> a = lane_from_left(right);
             const V4<R> fl = hll_flux_v2<0, M, R, E>(a, lane_from_left(v2r), L.phi[ty + 1][tx], b, v2l, f0.x, P);
             if (tx != 0) L.fx[ty][tx] = fl;
@@ -568,21 +574,32 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             }
             const V4<R> fxm = L.fx[ty][tx], fxp = L.fx[ty][tx + 1];
             const V4<R> fym = L.fy[ty][tx], fyp = L.fy[ty + 1][tx];
-            V4<R> fd{(fxp.x - fxm.x) * P.inv_dx, (fxp.y - fxm.y) * P.inv_dx, (fxp.z - fxm.z) * P.inv_dx,
-                     (fxp.w - fxm.w) * P.inv_dx};
-            if constexpr (!M::fast) fd = V4<R>{R(0) + fd.x, R(0) + fd.y, R(0) + fd.z, R(0) + fd.w};
-            fd.x = fd.x + (fyp.x - fym.x) * P.inv_dx;
-            fd.y = fd.y + (fyp.y - fym.y) * P.inv_dx;
-            fd.z = fd.z + (fyp.z - fym.z) * P.inv_dx;
-            fd.w = fd.w + (fyp.w - fym.w) * P.inv_dx;
-            fd.x = fd.x + (fzp.x - fzm.x) * P.inv_dx;
-            fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
-            fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
-            fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
             R phi_o, pi_o;
             V4<R> fo;
-            stage_update<M, R, S0>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, fnn.x, fnn.y, un, a0, a1, dt, P,
-                                   phi_o, pi_o, fo);
+            if constexpr (M::fast) {
+                // the three flux differences summed unscaled; 1/dx is applied
+                // with dt in the update
+                const V4<R> fs{(fxp.x - fxm.x) + (fyp.x - fym.x) + (fzp.x - fzm.x),
+                               (fxp.y - fxm.y) + (fyp.y - fym.y) + (fzp.y - fzm.y),
+                               (fxp.z - fxm.z) + (fyp.z - fym.z) + (fzp.z - fzm.z),
+                               (fxp.w - fxm.w) + (fyp.w - fym.w) + (fzp.w - fzm.w)};
+                stage_update<M, R, S0, true>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fs, uin, fnn.x, fnn.y, un, a0, a1,
+                                             dt, P, phi_o, pi_o, fo);
+            } else {
+                V4<R> fd{(fxp.x - fxm.x) * P.inv_dx, (fxp.y - fxm.y) * P.inv_dx, (fxp.z - fxm.z) * P.inv_dx,
+                         (fxp.w - fxm.w) * P.inv_dx};
+                fd = V4<R>{R(0) + fd.x, R(0) + fd.y, R(0) + fd.z, R(0) + fd.w};
+                fd.x = fd.x + (fyp.x - fym.x) * P.inv_dx;
+                fd.y = fd.y + (fyp.y - fym.y) * P.inv_dx;
+                fd.z = fd.z + (fyp.z - fym.z) * P.inv_dx;
+                fd.w = fd.w + (fyp.w - fym.w) * P.inv_dx;
+                fd.x = fd.x + (fzp.x - fzm.x) * P.inv_dx;
+                fd.y = fd.y + (fzp.y - fzm.y) * P.inv_dx;
+                fd.z = fd.z + (fzp.z - fzm.z) * P.inv_dx;
+                fd.w = fd.w + (fzp.w - fzm.w) * P.inv_dx;
+                stage_update<M, R, S0>(ph_0, pi0, qz, lap_sum, gxv, gyv, gzv, fd, uin, fnn.x, fnn.y, un, a0, a1, dt, P,
+                                       phi_o, pi_o, fo);
+            }
             stream_store(field_at(field_out, z), col, V2<R>{phi_o, pi_o});
             stream_store(fluid_at(fluid_out, z), col, fo);
         }

@@ -246,28 +246,28 @@ __host__ __device__ inline V4<R> eos_prim_from_cons(V4<R> u, R phi, const Params
     R zx = u.y, zy = u.z, zz = u.w;
     const R z2raw = dot3(zx, zy, zz, zx, zy, zz);
     const R zcap = P.z_max * e_lab;
-    R z2;
     if constexpr (M::fast) {
-        // |Z| > zcap without the square root (zcap > 0); the rescale is
-        // branch-free, by 1 when nothing is clamped, and |Z|^2 after it is
-        // zcap^2 itself
-        const R zcap2 = zcap * zcap;
-        const bool clamp = z2raw > zcap2;
-        const R f = clamp ? zcap * M::inv_sqrt(z2raw) : R(1);
-        zx = zx * f; zy = zy * f; zz = zz * f;
-        z2 = clamp ? zcap2 : z2raw;
+        // The cap without a square root or a branch (zcap > 0): the rescale
+        // factor is min(1, zcap / |Z|) (1 for Z = 0, where the reciprocal
+        // square root is +inf), and |Z|^2 after it is min(|Z|^2, zcap^2).
+        // The factor joins the reciprocal of E + p, so v = Z (f / (E + p)):
+        // one product per component.
+        const R f = M::min(R(1), zcap * M::inv_sqrt(z2raw));
+        const R z2 = M::min(z2raw, zcap * zcap);
+        const R p = E::template pressure_from_conserved<M>(e_lab, z2, phi, P);
+        const R rs = f * M::div(R(1), e_lab + p);  // E + p >= e_floor + p_floor > 0
+        return V4<R>{zx * rs, zy * rs, zz * rs, p};
     } else {
         if (M::sqrt(z2raw) > zcap) {
             const R zlen = M::sqrt(z2raw);
             const R f = M::div(zcap, M::max(zlen, Consts<R>::tiny));
             zx = zx * f; zy = zy * f; zz = zz * f;
         }
-        z2 = dot3(zx, zy, zz, zx, zy, zz);
+        const R z2 = dot3(zx, zy, zz, zx, zy, zz);
+        const R p = E::template pressure_from_conserved<M>(e_lab, z2, phi, P);
+        const R sm = M::max(e_lab + p, Consts<R>::tiny);
+        return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
     }
-    const R p = E::template pressure_from_conserved<M>(e_lab, z2, phi, P);
-    const R s = e_lab + p;
-    const R sm = M::fast ? s : M::max(s, Consts<R>::tiny);  // s >= e_floor + p_floor > 0
-    return V4<R>{M::div(zx, sm), M::div(zy, sm), M::div(zz, sm), p};
 }
 
 // eos_cons_from_prim (eos.wgsl:129-135).
@@ -338,6 +338,33 @@ __host__ __device__ inline V4<R> sanitise_v2(const V4<R> &q, const Params<R> &P,
     return V4<R>{vx, vy, vz, p};
 }
 
+// sanitise_v2 of the two states of one face.  FastMath tests both against
+// the cap in one branch, which no wave takes unless one of its states is
+// clamped; inside it each state is scaled by its own factor (1 when it is
+// not clamped), so every value is the one sanitise_v2 gives.
+template <class M, class R>
+__host__ __device__ inline void sanitise_pair_v2(V4<R> &a, R &v2a, V4<R> &b, R &v2b, const Params<R> &P) {
+    if constexpr (M::fast) {
+        const R vmax2 = P.z_max * P.z_max;
+        a.w = M::max(a.w, P.p_floor);
+        b.w = M::max(b.w, P.p_floor);
+        v2a = dot3(a.x, a.y, a.z, a.x, a.y, a.z);
+        v2b = dot3(b.x, b.y, b.z, b.x, b.y, b.z);
+        const bool ca = v2a > vmax2, cb = v2b > vmax2;
+        if (ca | cb) {
+            const R fa = ca ? M::sqrt(M::div(vmax2, M::max(v2a, Consts<R>::tiny))) : R(1);
+            const R fb = cb ? M::sqrt(M::div(vmax2, M::max(v2b, Consts<R>::tiny))) : R(1);
+            a.x = a.x * fa; a.y = a.y * fa; a.z = a.z * fa;
+            b.x = b.x * fb; b.y = b.y * fb; b.z = b.z * fb;
+            if (ca) v2a = vmax2;
+            if (cb) v2b = vmax2;
+        }
+    } else {
+        a = sanitise_v2<M>(a, P, v2a);
+        b = sanitise_v2<M>(b, P, v2b);
+    }
+}
+
 // |v|^2 capped at z_max^2 (eos.wgsl:130, 166).  The FastMath sanitise_v2
 // already returns a value <= z_max^2 (z_max^2 itself when it clamps), so the
 // cap is the identity there and is left out.
@@ -373,12 +400,21 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
     const R cs2 = E::sound_speed2(q.w, phi, P);
     const R denom = R(1) - v2 * cs2;
     if constexpr (M::fast) {
-        // cs sqrt(rad) as sqrt(cs^2 rad); rad > 0 without the clamp, since
-        // v_d^2 <= |v|^2 <= z_max^2 < 1 after sanitise
-        const R rad_c = (R(1) - v2) * ((R(1) - vd * vd - cs2 * (v2 - vd * vd)) * cs2);
-        const R rd = M::div(R(1), denom);
-        const R a = vd * (R(1) - cs2) * rd;
-        const R b = M::sqrt(rad_c) * rd;
+        // The same speeds with g = 1 / (1 - c^2) taken into the denominator:
+        // D = g (1 - v^2 c^2) = g - g c^2 v^2 and
+        // 1 - v_d^2 - c^2 (v^2 - v_d^2) = (1 - c^2) (D - v_d^2), so
+        //   lambda+- = (v_d +- sqrt(g c^2 (1 - v^2) (D - v_d^2))) / D.
+        // D - v_d^2 >= g (1 - v^2) > 0 (v_d^2 <= |v|^2 <= z_max^2 < 1 after
+        // sanitise), so the radicand needs no clamp.  g and g c^2 depend on
+        // the EOS constants only (1.5 and 0.5 for the bag model).
+        (void)denom;
+        const R g = M::div(R(1), R(1) - cs2);
+        const R gc2 = g * cs2;
+        const R d = g - gc2 * v2;
+        const R rad = ((R(1) - v2) * (d - vd * vd)) * gc2;
+        const R rd = M::div(R(1), d);
+        const R a = vd * rd;
+        const R b = M::sqrt(rad) * rd;
         lm = a - b;
         lp = a + b;
     } else {
@@ -504,8 +540,8 @@ __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, co
     const V4<R> hl = half_slope4<M>(sub4(ql, qlm), d);
     const V4<R> hr = half_slope4<M>(d, sub4(qrp, qr));
     R v2l, v2r;
-    const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
-    const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
+    V4<R> a = add4(ql, hl), b = sub4(qr, hr);
+    sanitise_pair_v2<M>(a, v2l, b, v2r, P);
     return hll_flux_v2<A, M, R, E>(a, v2l, phil, b, v2r, phir, P);
 }
 // Same, with the two half slopes (0.5 * limit_slope) already known.
@@ -513,8 +549,8 @@ template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, const V4<R> &qr,
                                              const V4<R> &hr, R phil, R phir, const Params<R> &P) {
     R v2l, v2r;
-    const V4<R> a = sanitise_v2<M>(add4(ql, hl), P, v2l);
-    const V4<R> b = sanitise_v2<M>(sub4(qr, hr), P, v2r);
+    V4<R> a = add4(ql, hl), b = sub4(qr, hr);
+    sanitise_pair_v2<M>(a, v2l, b, v2r, P);
     return hll_flux_v2<A, M, R, E>(a, v2l, phil, b, v2r, phir, P);
 }
 
@@ -527,7 +563,11 @@ __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, c
 // S0: the first stage (a0 = 0, a1 = 1), where FastMath takes the combination
 // as U_in + dt RHS itself; the exact policy keeps 0 * U^n + 1 * (...), whose
 // zero signs follow the reference.
-template <class M, class R, bool S0 = false>
+// RAW (FastMath only): flux_div holds the unscaled sum of the three flux
+// differences, and 1/dx joins dt in one factor, as 1/dx^2 does for the
+// Laplacian: U_in + dt (drag terms) - (dt/dx) sum, each component two
+// multiply-adds.
+template <class M, class R, bool S0 = false, bool RAW = false>
 __host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R lap_sum, R gx, R gy, R gz,
                                              const V4<R> &flux_div, const V4<R> &fin,
                                              R phin, R pin, const V4<R> &fn,
@@ -538,15 +578,28 @@ __host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R l
     const R dphi_proper = gam * (pi0 + dot3(q0.x, q0.y, q0.z, gx, gy, gz));
     const R drag = P.eta * dphi_proper;
     const R d_phi = pi0;
-    const R d_pi = laplacian - dpotential(phi0, P) - drag;
-    V4<R> d{-flux_div.x, -flux_div.y, -flux_div.z, -flux_div.w};
-    d.x = d.x + drag * pi0;
-    d.y = d.y - drag * gx;
-    d.z = d.z - drag * gy;
-    d.w = d.w - drag * gz;
-    const R up_phi = phi0 + dt * d_phi;
-    const R up_pi = pi0 + dt * d_pi;
-    const V4<R> up{fin.x + dt * d.x, fin.y + dt * d.y, fin.z + dt * d.z, fin.w + dt * d.w};
+    R up_phi, up_pi;
+    V4<R> up;
+    if constexpr (RAW) {
+        static_assert(M::fast, "the unscaled divergence is a FastMath form");
+        (void)laplacian;
+        const R dtd = dt * drag;
+        const R dtx = dt * P.inv_dx;
+        up_phi = phi0 + dt * d_phi;
+        up_pi = (pi0 - dt * (dpotential(phi0, P) + drag)) + (dt * (P.inv_dx * P.inv_dx)) * lap_sum;
+        up = V4<R>{(fin.x + dtd * pi0) - dtx * flux_div.x, (fin.y - dtd * gx) - dtx * flux_div.y,
+                   (fin.z - dtd * gy) - dtx * flux_div.z, (fin.w - dtd * gz) - dtx * flux_div.w};
+    } else {
+        const R d_pi = laplacian - dpotential(phi0, P) - drag;
+        V4<R> d{-flux_div.x, -flux_div.y, -flux_div.z, -flux_div.w};
+        d.x = d.x + drag * pi0;
+        d.y = d.y - drag * gx;
+        d.z = d.z - drag * gy;
+        d.w = d.w - drag * gz;
+        up_phi = phi0 + dt * d_phi;
+        up_pi = pi0 + dt * d_pi;
+        up = V4<R>{fin.x + dt * d.x, fin.y + dt * d.y, fin.z + dt * d.z, fin.w + dt * d.w};
+    }
     if constexpr (M::fast && S0) {
         phi_out = up_phi;
         pi_out = up_pi;
