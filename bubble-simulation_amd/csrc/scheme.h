@@ -166,6 +166,10 @@ struct BagEos {
     template <class M, class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &) {
         return R(4) * p;
     }
+    // w / p (the FastMath flux folds it into one loop-invariant factor)
+    template <class M, class R> __host__ __device__ static inline R enthalpy_per_pressure(R /*phi*/, const Params<R> &) {
+        return R(4);
+    }
     // eos_energy_from_pressure (eos.wgsl:61-63): e = 3p.
     template <class R> __host__ __device__ static inline R energy(R p, R /*phi*/, const Params<R> &) {
         return R(3) * p;
@@ -202,6 +206,9 @@ struct GammaLawEos {
     template <class M, class R> __host__ __device__ static inline R enthalpy(R p, R /*phi*/, const Params<R> &P) {
         if constexpr (M::fast) return p * M::div(P.eos_gamma, P.eos_gamma - R(1));
         else return P.eos_gamma * p / (P.eos_gamma - R(1));
+    }
+    template <class M, class R> __host__ __device__ static inline R enthalpy_per_pressure(R /*phi*/, const Params<R> &P) {
+        return M::div(P.eos_gamma, P.eos_gamma - R(1));
     }
     template <class R> __host__ __device__ static inline R energy(R p, R /*phi*/, const Params<R> &P) {
         return p / (P.eos_gamma - R(1));
@@ -399,32 +406,42 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
     const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
     const R cs2 = E::sound_speed2(q.w, phi, P);
     const R denom = R(1) - v2 * cs2;
-    if constexpr (M::fast) {
-        // The same speeds with g = 1 / (1 - c^2) taken into the denominator:
-        // D = g (1 - v^2 c^2) = g - g c^2 v^2 and
-        // 1 - v_d^2 - c^2 (v^2 - v_d^2) = (1 - c^2) (D - v_d^2), so
-        //   lambda+- = (v_d +- sqrt(g c^2 (1 - v^2) (D - v_d^2))) / D.
-        // D - v_d^2 >= g (1 - v^2) > 0 (v_d^2 <= |v|^2 <= z_max^2 < 1 after
-        // sanitise), so the radicand needs no clamp.  g and g c^2 depend on
-        // the EOS constants only (1.5 and 0.5 for the bag model).
-        (void)denom;
-        const R g = M::div(R(1), R(1) - cs2);
-        const R gc2 = g * cs2;
-        const R d = g - gc2 * v2;
-        const R rad = ((R(1) - v2) * (d - vd * vd)) * gc2;
-        const R rd = M::div(R(1), d);
-        const R a = vd * rd;
-        const R b = M::sqrt(rad) * rd;
-        lm = a - b;
-        lp = a + b;
-    } else {
-        const R cs = M::sqrt(cs2);
-        const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
-        const R a = M::div(vd * (R(1) - cs2), denom);
-        const R b = M::div(cs * M::sqrt(rad), denom);
-        lm = a - b;
-        lp = a + b;
-    }
+    const R cs = M::sqrt(cs2);
+    const R rad = M::max((R(1) - v2) * (R(1) - vd * vd - cs2 * (v2 - vd * vd)), R(0));
+    const R a = M::div(vd * (R(1) - cs2), denom);
+    const R b = M::div(cs * M::sqrt(rad), denom);
+    lm = a - b;
+    lp = a + b;
+}
+
+// FastMath: what the HLL flux needs of one face state -- w gamma^2 and the
+// wave speeds (eos_cons_from_prim, eos_wave_speeds: eos.wgsl:129-135,
+// 164-176) -- with the constants of the equation of state taken into the
+// factors.  With g = 1 / (1 - c^2), o = g c^2 (1 - v^2) and
+// D = g (1 - v^2 c^2) = g - g c^2 v^2, and since
+// 1 - v_d^2 - c^2 (v^2 - v_d^2) = (1 - c^2) (D - v_d^2):
+//   lambda+- = (v_d +- sqrt(o (D - v_d^2))) / D,
+//   w gamma^2 = (w / p) p / (1 - v^2) = ((w / p) g c^2) p / o.
+// D - v_d^2 >= g (1 - v^2) > 0 (v_d^2 <= |v|^2 <= z_max^2 < 1 after
+// sanitise), so the radicand needs no clamp.  g, g c^2 and the enthalpy
+// factor depend on the EOS constants only (1.5, 0.5 and 4 x 0.5 for the bag
+// model), so they are formed once per launch.
+template <int A, class M, class R, class E = BagEos>
+__host__ __device__ inline void flux_side_fast(const V4<R> &q, R v2, R phi, const Params<R> &P, R &w_gam2, R &lm, R &lp) {
+    static_assert(M::fast, "the factored side terms are a FastMath form");
+    const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
+    const R cs2 = E::sound_speed2(q.w, phi, P);
+    const R g = M::div(R(1), R(1) - cs2);
+    const R gc2 = g * cs2;
+    const R o = gc2 - gc2 * v2;
+    const R d = g - gc2 * v2;
+    const R rd = M::div(R(1), d);
+    const R a = vd * rd;
+    const R b = M::sqrt(o * (d - vd * vd)) * rd;
+    lm = a - b;
+    lp = a + b;
+    const R hk = E::template enthalpy_per_pressure<M>(phi, P) * gc2;
+    w_gam2 = (q.w * hk) * M::div(R(1), o);
 }
 
 // hll_flux (step.wgsl:77-90) along axis A, with |v|^2 of both states given
@@ -442,24 +459,25 @@ template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
     if constexpr (M::fast) {
-        const R wl = M::div(E::template enthalpy<M>(ql.w, phil, P), R(1) - capped_v2<M>(v2l, P));
-        const R wr = M::div(E::template enthalpy<M>(qr.w, phir, P), R(1) - capped_v2<M>(v2r, P));
-        R llm, llp, lrm, lrp;
-        eos_wave_speeds_v2<A, M, R, E>(ql, v2l, phil, P, llm, llp);
-        eos_wave_speeds_v2<A, M, R, E>(qr, v2r, phir, P, lrm, lrp);
+        R wl, llm, llp, wr, lrm, lrp;
+        flux_side_fast<A, M, R, E>(ql, v2l, phil, P, wl, llm, llp);
+        flux_side_fast<A, M, R, E>(qr, v2r, phir, P, wr, lrm, lrp);
         const R s_min = M::min3(llm, lrm, R(0));
         const R s_max = M::max3(llp, lrp, R(0));
         const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
         const R cmax = s_max * rden, cmin = s_min * rden;
         const R vdl = A == 0 ? ql.x : (A == 1 ? ql.y : ql.z);
         const R vdr = A == 0 ? qr.x : (A == 1 ? qr.y : qr.z);
-        const R tl = wl * (vdl - s_min), tr = wr * (vdr - s_max);
-        const V4<R> a{tl + s_min * ql.w, ql.x * tl + (A == 0 ? ql.w : R(0)), ql.y * tl + (A == 1 ? ql.w : R(0)),
-                      ql.z * tl + (A == 2 ? ql.w : R(0))};
-        const V4<R> b{tr + s_max * qr.w, qr.x * tr + (A == 0 ? qr.w : R(0)), qr.y * tr + (A == 1 ? qr.w : R(0)),
-                      qr.z * tr + (A == 2 ? qr.w : R(0))};
-        return V4<R>{cmax * a.x - cmin * b.x, cmax * a.y - cmin * b.y, cmax * a.z - cmin * b.z,
-                     cmax * a.w - cmin * b.w};
+        // F = cmax A - cmin B, collected by term: al = cmax t_L, be = cmin t_R,
+        // and cmax s- = cmin s+ = s- s+ / (s+ - s-) multiplies both pressures
+        const R al = (wl * (vdl - s_min)) * cmax;
+        const R be = (wr * (vdr - s_max)) * cmin;
+        const R mu = s_min * cmax;
+        const R kp = cmax * ql.w - cmin * qr.w;
+        return V4<R>{(al - be) + mu * (ql.w - qr.w),
+                     al * ql.x + ((A == 0 ? kp : R(0)) - be * qr.x),
+                     al * ql.y + ((A == 1 ? kp : R(0)) - be * qr.y),
+                     al * ql.z + ((A == 2 ? kp : R(0)) - be * qr.z)};
     } else {
         R wl, wr;
         const V4<R> ul = eos_cons_from_prim_v2<M, R, E>(ql, v2l, phil, P, wl);
