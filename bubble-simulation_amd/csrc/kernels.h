@@ -365,13 +365,13 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
     }
     // faces: round 1 gives every thread the x face left of its cell, round 2
     // the y face below it; the TY + TX faces on the far edges (x = TX, y = TY)
-    // form one more round in which the y faces are evaluated as x faces of
-    // the component-swapped states (vx <-> vy), so that round is one uniform
-    // code path instead of one per axis.  The swap commutes with every
-    // operation of the face (the sums of squares add the same two products
-    // in either order), so under ExactMath the result is the y face's, bit
-    // for bit; under FastMath, FMA contraction rounds vx^2 + vy^2 in the
-    // other order, so those faces agree to rounding.
+    // form one more round, one uniform code path for both directions.  Under
+    // ExactMath the y faces are evaluated as x faces of the component-swapped
+    // states (vx <-> vy): the swap commutes with every operation of the face
+    // (the sums of squares add the same two products in either order), so
+    // the result is the y face's, bit for bit.  FastMath takes the direction
+    // as a per-lane value instead (a select for the normal velocity and one
+    // for the pressure term, where the swap moved every component).
     // FastMath shares each cell's x half slope with the face on its left:
     // the cell forms q - h (the right state of its left face) and q + h (the
     // left state of its right face), and the lane to its right takes the
@@ -530,12 +530,20 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             const int p0 = ya ? TY * (TX + 2) + (c + 1) : (c + 1) * (TX + 2) + xe;
             const int ps = ya ? TX + 2 : 1;
             V4<R> c0 = qflat[q0], c1 = qflat[q0 + qs], c2 = qflat[q0 + 2 * qs], c3 = qflat[q0 + 3 * qs];
-            if (ya) {
-                c0 = swap_xy(c0); c1 = swap_xy(c1); c2 = swap_xy(c2); c3 = swap_xy(c3);
+            V4<R> ft;
+            if constexpr (M::fast) {
+                // the face direction as a per-lane value: the normal velocity
+                // and the pressure term take a select each
+                ft = face_flux_fast_xy<M, R, E>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P, ya ? 1 : 0);
+            } else {
+                if (ya) {
+                    c0 = swap_xy(c0); c1 = swap_xy(c1); c2 = swap_xy(c2); c3 = swap_xy(c3);
+                }
+                ft = face_flux<0, M, R, E>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P);
+                if (ya) ft = swap_yz(ft);
             }
-            V4<R> ft = face_flux<0, M, R, E>(c0, c1, c2, c3, phiflat[p0], phiflat[p0 + ps], P);
             if (ya) {
-                L.fy[TY][c] = swap_yz(ft);
+                L.fy[TY][c] = ft;
             } else {
                 L.fx[c][xe] = ft;
             }
@@ -557,12 +565,12 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
             const R ph_0 = f0.x, ph_1 = f1.x;
             R lap_sum, gxv, gyv, gzv;
             if constexpr (M::fast) {
-                // the same sums regrouped, and 0.5 / dx as one factor
+                // the same sums regrouped; the central differences are
+                // scaled by 1/(2 dx) inside the update
                 lap_sum = ((pxm + pxp) + (pym + pyp)) + (ph_m + ph_1) - R(6) * ph_0;
-                const R hdx = R(0.5) * P.inv_dx;
-                gxv = (pxp - pxm) * hdx;
-                gyv = (pyp - pym) * hdx;
-                gzv = (ph_1 - ph_m) * hdx;
+                gxv = pxp - pxm;
+                gyv = pyp - pym;
+                gzv = ph_1 - ph_m;
             } else {
                 // the reference accumulates from zero (step.wgsl:117-145)
                 lap_sum = R(0) + (pxm + pxp - R(2) * ph_0);

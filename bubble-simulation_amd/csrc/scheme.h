@@ -414,8 +414,8 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
     lp = a + b;
 }
 
-// FastMath: what the HLL flux needs of one face state -- w gamma^2 and the
-// wave speeds (eos_cons_from_prim, eos_wave_speeds: eos.wgsl:129-135,
+// FastMath: what the HLL flux needs of one face state, given its velocity
+// component v_d normal to the face -- w gamma^2 and the wave speeds (eos_cons_from_prim, eos_wave_speeds: eos.wgsl:129-135,
 // 164-176) -- with the constants of the equation of state taken into the
 // factors.  With g = 1 / (1 - c^2), o = g c^2 (1 - v^2) and
 // D = g (1 - v^2 c^2) = g - g c^2 v^2, and since
@@ -426,10 +426,10 @@ __host__ __device__ inline void eos_wave_speeds_v2(const V4<R> &q, R v2raw, R ph
 // sanitise), so the radicand needs no clamp.  g, g c^2 and the enthalpy
 // factor depend on the EOS constants only (1.5, 0.5 and 4 x 0.5 for the bag
 // model), so they are formed once per launch.
-template <int A, class M, class R, class E = BagEos>
-__host__ __device__ inline void flux_side_fast(const V4<R> &q, R v2, R phi, const Params<R> &P, R &w_gam2, R &lm, R &lp) {
+template <class M, class R, class E = BagEos>
+__host__ __device__ inline void flux_side_fast(const V4<R> &q, R vd, R v2, R phi, const Params<R> &P, R &w_gam2,
+                                               R &lm, R &lp) {
     static_assert(M::fast, "the factored side terms are a FastMath form");
-    const R vd = A == 0 ? q.x : (A == 1 ? q.y : q.z);
     const R cs2 = E::sound_speed2(q.w, phi, P);
     const R g = M::div(R(1), R(1) - cs2);
     const R gc2 = g * cs2;
@@ -442,6 +442,35 @@ __host__ __device__ inline void flux_side_fast(const V4<R> &q, R v2, R phi, cons
     lp = a + b;
     const R hk = E::template enthalpy_per_pressure<M>(phi, P) * gc2;
     w_gam2 = (q.w * hk) * M::div(R(1), o);
+}
+
+// FastMath HLL flux along `axis` (0, 1, 2): a constant where the face
+// direction is known at compile time, a per-lane value where one code path
+// evaluates faces of two directions (the normal velocity and the pressure
+// term then take one select each).  The flux is collected by term:
+// F = cmax A - cmin B with al = cmax t_L, be = cmin t_R, and
+// cmax s- = cmin s+ = s- s+ / (s+ - s-) multiplying both pressures.
+template <class M, class R, class E = BagEos>
+__host__ __device__ inline V4<R> hll_flux_fast(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
+                                               const Params<R> &P, int axis) {
+    static_assert(M::fast, "the factored flux is a FastMath form");
+    const R vdl = axis == 0 ? ql.x : (axis == 1 ? ql.y : ql.z);
+    const R vdr = axis == 0 ? qr.x : (axis == 1 ? qr.y : qr.z);
+    R wl, llm, llp, wr, lrm, lrp;
+    flux_side_fast<M, R, E>(ql, vdl, v2l, phil, P, wl, llm, llp);
+    flux_side_fast<M, R, E>(qr, vdr, v2r, phir, P, wr, lrm, lrp);
+    const R s_min = M::min3(llm, lrm, R(0));
+    const R s_max = M::max3(llp, lrp, R(0));
+    const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
+    const R cmax = s_max * rden, cmin = s_min * rden;
+    const R al = (wl * (vdl - s_min)) * cmax;
+    const R be = (wr * (vdr - s_max)) * cmin;
+    const R mu = s_min * cmax;
+    const R kp = cmax * ql.w - cmin * qr.w;
+    return V4<R>{(al - be) + mu * (ql.w - qr.w),
+                 al * ql.x + ((axis == 0 ? kp : R(0)) - be * qr.x),
+                 al * ql.y + ((axis == 1 ? kp : R(0)) - be * qr.y),
+                 al * ql.z + ((axis == 2 ? kp : R(0)) - be * qr.z)};
 }
 
 // hll_flux (step.wgsl:77-90) along axis A, with |v|^2 of both states given
@@ -459,25 +488,7 @@ template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> hll_flux_v2(const V4<R> &ql, R v2l, R phil, const V4<R> &qr, R v2r, R phir,
                                              const Params<R> &P) {
     if constexpr (M::fast) {
-        R wl, llm, llp, wr, lrm, lrp;
-        flux_side_fast<A, M, R, E>(ql, v2l, phil, P, wl, llm, llp);
-        flux_side_fast<A, M, R, E>(qr, v2r, phir, P, wr, lrm, lrp);
-        const R s_min = M::min3(llm, lrm, R(0));
-        const R s_max = M::max3(llp, lrp, R(0));
-        const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
-        const R cmax = s_max * rden, cmin = s_min * rden;
-        const R vdl = A == 0 ? ql.x : (A == 1 ? ql.y : ql.z);
-        const R vdr = A == 0 ? qr.x : (A == 1 ? qr.y : qr.z);
-        // F = cmax A - cmin B, collected by term: al = cmax t_L, be = cmin t_R,
-        // and cmax s- = cmin s+ = s- s+ / (s+ - s-) multiplies both pressures
-        const R al = (wl * (vdl - s_min)) * cmax;
-        const R be = (wr * (vdr - s_max)) * cmin;
-        const R mu = s_min * cmax;
-        const R kp = cmax * ql.w - cmin * qr.w;
-        return V4<R>{(al - be) + mu * (ql.w - qr.w),
-                     al * ql.x + ((A == 0 ? kp : R(0)) - be * qr.x),
-                     al * ql.y + ((A == 1 ? kp : R(0)) - be * qr.y),
-                     al * ql.z + ((A == 2 ? kp : R(0)) - be * qr.z)};
+        return hll_flux_fast<M, R, E>(ql, v2l, phil, qr, v2r, phir, P, A);
     } else {
         R wl, wr;
         const V4<R> ul = eos_cons_from_prim_v2<M, R, E>(ql, v2l, phil, P, wl);
@@ -562,6 +573,19 @@ __host__ __device__ inline V4<R> face_flux(const V4<R> &qlm, const V4<R> &ql, co
     sanitise_pair_v2<M>(a, v2l, b, v2r, P);
     return hll_flux_v2<A, M, R, E>(a, v2l, phil, b, v2r, phir, P);
 }
+// face_flux under FastMath with the face direction as a per-lane value
+// (0: x, 1: y), for a code path that evaluates faces of both directions
+template <class M, class R, class E = BagEos>
+__host__ __device__ inline V4<R> face_flux_fast_xy(const V4<R> &qlm, const V4<R> &ql, const V4<R> &qr,
+                                                   const V4<R> &qrp, R phil, R phir, const Params<R> &P, int axis) {
+    const V4<R> d = sub4(qr, ql);
+    const V4<R> hl = half_slope4<M>(sub4(ql, qlm), d);
+    const V4<R> hr = half_slope4<M>(d, sub4(qrp, qr));
+    R v2l, v2r;
+    V4<R> a = add4(ql, hl), b = sub4(qr, hr);
+    sanitise_pair_v2<M>(a, v2l, b, v2r, P);
+    return hll_flux_fast<M, R, E>(a, v2l, phil, b, v2r, phir, P, axis);
+}
 // Same, with the two half slopes (0.5 * limit_slope) already known.
 template <int A, class M, class R, class E = BagEos>
 __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, const V4<R> &qr,
@@ -582,32 +606,40 @@ __host__ __device__ inline V4<R> face_flux_h(const V4<R> &ql, const V4<R> &hl, c
 // as U_in + dt RHS itself; the exact policy keeps 0 * U^n + 1 * (...), whose
 // zero signs follow the reference.
 // RAW (FastMath only): flux_div holds the unscaled sum of the three flux
-// differences, and 1/dx joins dt in one factor, as 1/dx^2 does for the
-// Laplacian: U_in + dt (drag terms) - (dt/dx) sum, each component two
-// multiply-adds.
+// differences and gx, gy, gz the unscaled central differences of phi (2 dx
+// times the gradient); 1/dx joins dt in one factor, as 1/dx^2 does for the
+// Laplacian, and 1/(2 dx) joins the drag: U_in + dt (drag terms) - (dt/dx)
+// sum, each component two multiply-adds.
 template <class M, class R, bool S0 = false, bool RAW = false>
 __host__ __device__ inline void stage_update(R phi0, R pi0, const V4<R> &q0, R lap_sum, R gx, R gy, R gz,
                                              const V4<R> &flux_div, const V4<R> &fin,
                                              R phin, R pin, const V4<R> &fn,
                                              R a0, R a1, R dt, const Params<R> &P,
                                              R &phi_out, R &pi_out, V4<R> &f_out) {
-    const R laplacian = M::fast ? lap_sum * (P.inv_dx * P.inv_dx) : lap_sum * P.inv_dx * P.inv_dx;
-    const R gam = M::inv_sqrt(M::max(R(1) - dot3(q0.x, q0.y, q0.z, q0.x, q0.y, q0.z), Consts<R>::den_floor));
-    const R dphi_proper = gam * (pi0 + dot3(q0.x, q0.y, q0.z, gx, gy, gz));
-    const R drag = P.eta * dphi_proper;
     const R d_phi = pi0;
     R up_phi, up_pi;
     V4<R> up;
     if constexpr (RAW) {
         static_assert(M::fast, "the unscaled divergence is a FastMath form");
-        (void)laplacian;
+        // |v| < z_max for a recovered primitive (the recovery caps |Z| at
+        // z_max E and divides by E + p > E), so 1 - v^2 >= 1 - z_max^2 needs
+        // no floor here
+        const R hdx = R(0.5) * P.inv_dx;
+        const R gam = M::inv_sqrt(R(1) - dot3(q0.x, q0.y, q0.z, q0.x, q0.y, q0.z));
+        const R dphi_proper = gam * (pi0 + hdx * dot3(q0.x, q0.y, q0.z, gx, gy, gz));
+        const R drag = P.eta * dphi_proper;
         const R dtd = dt * drag;
+        const R dtg = dtd * hdx;
         const R dtx = dt * P.inv_dx;
         up_phi = phi0 + dt * d_phi;
         up_pi = (pi0 - dt * (dpotential(phi0, P) + drag)) + (dt * (P.inv_dx * P.inv_dx)) * lap_sum;
-        up = V4<R>{(fin.x + dtd * pi0) - dtx * flux_div.x, (fin.y - dtd * gx) - dtx * flux_div.y,
-                   (fin.z - dtd * gy) - dtx * flux_div.z, (fin.w - dtd * gz) - dtx * flux_div.w};
+        up = V4<R>{(fin.x + dtd * pi0) - dtx * flux_div.x, (fin.y - dtg * gx) - dtx * flux_div.y,
+                   (fin.z - dtg * gy) - dtx * flux_div.z, (fin.w - dtg * gz) - dtx * flux_div.w};
     } else {
+        const R laplacian = M::fast ? lap_sum * (P.inv_dx * P.inv_dx) : lap_sum * P.inv_dx * P.inv_dx;
+        const R gam = M::inv_sqrt(M::max(R(1) - dot3(q0.x, q0.y, q0.z, q0.x, q0.y, q0.z), Consts<R>::den_floor));
+        const R dphi_proper = gam * (pi0 + dot3(q0.x, q0.y, q0.z, gx, gy, gz));
+        const R drag = P.eta * dphi_proper;
         const R d_pi = laplacian - dpotential(phi0, P) - drag;
         V4<R> d{-flux_div.x, -flux_div.y, -flux_div.z, -flux_div.w};
         d.x = d.x + drag * pi0;
