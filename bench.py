@@ -232,23 +232,24 @@ def accuracy_sample(specs):
     return idx
 
 
-def batched_sweep(bs, steps=1000, report_at=200):
+def batched_sweep(bs, steps=1000, report_at=200, use_graph=True):
     """64 uneven boxes in one launch per RK stage (hipGraph replay) against
     the same boxes solved one after another on the plain path, both as whole
     jobs of `steps` steps from the initial state.  The batch reports
     diagnostics of a sample of boxes (one per edge 32, 64, 96, 128) at step
     `report_at` (every bubble has been stamped by then) and at the end, as
     the headless run reports, inside its timed region; the same sample is
-    then run in f64 through the same engine for the field error and drift."""
+    then run in f64 through the same engine for the field error and drift.
+    use_graph=False steps eagerly (for a run under a profiler)."""
     specs = sweep_specs(bs)
     sample = accuracy_sample(specs)
     b = bs.Batch(specs)
     cells = b.total_cells()
     b.synchronize()
     t0 = time.perf_counter()
-    b.advance(report_at, use_graph=True)
+    b.advance(report_at, use_graph=use_graph)
     e0 = {i: b.diagnostics(i).mean_energy for i in sample}
-    b.advance(steps - report_at, use_graph=True)
+    b.advance(steps - report_at, use_graph=use_graph)
     b.synchronize()
     dt = time.perf_counter() - t0
     e1 = {i: b.diagnostics(i) for i in sample}
@@ -264,9 +265,9 @@ def batched_sweep(bs, steps=1000, report_at=200):
     # f64 of the sample through the same engine
     def sample_run(make):
         bb = bs.Batch([make(bs, specs[i]) for i in sample])
-        bb.advance(report_at, use_graph=True)
+        bb.advance(report_at, use_graph=use_graph)
         a0 = [bb.diagnostics(k).mean_energy for k in range(len(sample))]
-        bb.advance(steps - report_at, use_graph=True)
+        bb.advance(steps - report_at, use_graph=use_graph)
         a1 = [bb.diagnostics(k).mean_energy for k in range(len(sample))]
         st = [bb.download(k) for k in range(len(sample))]
         bb.close()
@@ -320,11 +321,11 @@ def served_jobs(bs, n_jobs, rate_hz, seed):
     return jobs
 
 
-def serve(bs, jobs, burst, keep=()):
+def serve(bs, jobs, burst, keep=(), flags=0):
     """Submit the jobs to the continuous-batching server under a 288 GB -
     16 GB budget, at their arrival times or (burst) all at once; returns
     (wall seconds, per-job results, server stats, kept states by job index)."""
-    srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20)
+    srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20, flags=flags)
     try:
         t0 = time.perf_counter()
         ids = []
@@ -342,14 +343,14 @@ def serve(bs, jobs, burst, keep=()):
     return wall, res, st, kept
 
 
-def served_accuracy(bs, jobs, keep, kept, res):
+def served_accuracy(bs, jobs, keep, kept, res, flags=0):
     """Field error and drift of sampled served jobs against the same jobs run
     in f64 ExactMath through a server; the drift of each is taken from the
     job's baseline diagnostics (first tick after its last nucleation) to its
     final ones."""
     def sample_run(precision, make):
         srv = bs.Server(budget_bytes=0, reserve_bytes=16 << 30, chunk_steps=20, precision=precision,
-                        math=bs.MATH_EXACT)
+                        math=bs.MATH_EXACT, flags=flags)
         try:
             ids = [srv.submit(make(bs, jobs[k][1]), jobs[k][2], keep_state=True) for k in keep]
             rr = [srv.wait(j, 600) for j in ids]
@@ -372,12 +373,13 @@ def served_accuracy(bs, jobs, keep, kept, res):
             "blocks": field_errors([kept[k] for k in keep], ref), "blocks_f32_exact": field_errors(exact32, ref)}
 
 
-def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
+def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99, flags=0):
     """The continuous-batching server on a stream of sweep jobs: Poisson
     arrivals at rate_hz (p50/p99 latency and aggregate throughput, which the
     arrival rate bounds), then the same jobs submitted at once (the server's
     capacity), both against sequential plain solves of the same jobs (whole
-    jobs from step 0, as the server runs them)."""
+    jobs from step 0, as the server runs them).  flags: server flags
+    (SERVER_NO_GRAPHS steps eagerly, for a run under a profiler)."""
     jobs = served_jobs(bs, n_jobs, rate_hz, seed)
     work = sum(spec.grid[0] * spec.grid[1] * spec.grid[2] * steps for (_, spec, steps) in jobs)
     # a sample for the accuracy check: the first job of each of four edges
@@ -387,10 +389,10 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
         if e not in seen and len(keep) < 4:
             seen.add(e)
             keep.append(k)
-    wall, res, st, kept = serve(bs, jobs, burst=False, keep=keep)
+    wall, res, st, kept = serve(bs, jobs, burst=False, keep=keep, flags=flags)
     lat = np.array([r.latency_s for r in res])
     served = work / wall
-    bwall, bres, _, _ = serve(bs, jobs, burst=True)
+    bwall, bres, _, _ = serve(bs, jobs, burst=True, flags=flags)
     blat = np.array([r.latency_s for r in bres])
     burst = work / bwall
     ok = all(r.status == 0 for r in res) and all(r.status == 0 for r in bres)
@@ -408,7 +410,7 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
     log(f"bench: served {n_jobs} jobs at {rate_hz} Hz: wall {wall:.1f} s, {served / 1e9:.2f} G/s, "
         f"p50 {np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s; all at once: wall {bwall:.1f} s, "
         f"{burst / 1e9:.2f} G/s; sequential {sequential / 1e9:.2f} G/s")
-    acc = served_accuracy(bs, jobs, keep, kept, res) if len(kept) == len(keep) else None
+    acc = served_accuracy(bs, jobs, keep, kept, res, flags) if len(kept) == len(keep) else None
     if acc:
         log("bench: served accuracy (sample edges " + ", ".join(str(j["edge"]) for j in acc["jobs"]) + "): rel L2 "
             + ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in acc["blocks"].items()) + "; f32 exact "
@@ -417,6 +419,7 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99):
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,
             "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": ok,
+            "graphs": not (flags & bs.SERVER_NO_GRAPHS),
             "burst": {"wall_s": bwall, "site_updates_per_s": burst, "speedup_vs_sequential": burst / sequential,
                       "p50_latency_s": float(np.percentile(blat, 50)),
                       "p99_latency_s": float(np.percentile(blat, 99))},
@@ -533,6 +536,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="step the batched sweep and the served stream eagerly, without graph capture (for a run "
+                         "under a profiler)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the f32-against-f64 field error and drift")
     ap.add_argument("--eos", choices=["bag", "gamma-law"], default="bag",
                     help="equation of state of the stepped box (the headline is the bag model)")
@@ -610,8 +616,10 @@ def main():
     hbm = side_measurement("HBM context", hbm_context, bs, achieved, traffic, avg_launch_s)
     acc = None if args.no_accuracy else side_measurement(
         "accuracy", accuracy, bs, n, eos, args.eos_gamma if eos else 4.0 / 3.0)
-    batch = None if args.no_batch else side_measurement("batched sweep", batched_sweep, bs)
-    served = None if args.no_serve else side_measurement("served stream", served_stream, bs)
+    batch = None if args.no_batch else side_measurement("batched sweep", batched_sweep, bs, 1000, 200,
+                                                        not args.no_graphs)
+    served = None if args.no_serve else side_measurement(
+        "served stream", served_stream, bs, 48, 6.0, 99, bs.SERVER_NO_GRAPHS if args.no_graphs else 0)
     cpu = None if args.no_cpu else side_measurement("CPU baseline", cpu_baseline, bs)
     out = {
         "metric": "RK3 site-updates/s at 512^3 f32 (fused stage kernel, % of HBM peak)",

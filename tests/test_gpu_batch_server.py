@@ -93,6 +93,22 @@ def test_server_alone_equals_mixed(bs, gpu_available):
     srv.close()
 
 
+def test_server_without_graphs_equals_graphs(bs, gpu_available):
+    # SERVER_NO_GRAPHS (eager stepping, for runs under a profiler) launches the
+    # same kernels as the captured graphs: the same bits per job
+    specs = _specs(bs, bs.MATH_FAST, n=6, seed=7)
+    steps = [30, 45, 20, 60, 35, 50]
+    sums = []
+    for flags in (0, bs.SERVER_NO_GRAPHS):
+        srv = bs.Server(budget_bytes=2 << 30, chunk_steps=5, flags=flags)
+        ids = [srv.submit(s, n, checksum=True) for s, n in zip(specs, steps)]
+        res = _wait_all(srv, ids)
+        assert all(res[j].status == 0 for j in ids)
+        sums.append([res[j].state_checksum for j in ids])
+        srv.close()
+    assert sums[0] == sums[1]
+
+
 def test_server_exact_equals_simulation(bs, gpu_available):
     specs = _specs(bs, bs.MATH_EXACT, n=3, seed=11)
     srv = bs.Server(budget_bytes=1 << 30, chunk_steps=4, math=bs.MATH_EXACT)
