@@ -461,7 +461,10 @@ __host__ __device__ inline V4<R> hll_flux_fast(const V4<R> &ql, R v2l, R phil, c
     flux_side_fast<M, R, E>(qr, vdr, v2r, phir, P, wr, lrm, lrp);
     const R s_min = M::min3(llm, lrm, R(0));
     const R s_max = M::max3(llp, lrp, R(0));
-    const R rden = M::div(R(1), M::max(s_max - s_min, Consts<R>::den_floor));
+    // s+ - s- >= lambda+ - lambda- of either state = 2 sqrt(o (D - v_d^2)) / D,
+    // at least ~1e-7 for any sanitised state (|v| <= z_max, 0 < c^2 < 1),
+    // so the reference's 1e-12 floor on the denominator never binds here
+    const R rden = M::div(R(1), s_max - s_min);
     const R cmax = s_max * rden, cmin = s_min * rden;
     const R al = (wl * (vdl - s_min)) * cmax;
     const R be = (wr * (vdr - s_max)) * cmin;
