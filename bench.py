@@ -599,8 +599,12 @@ def main():
     early = window_rate(args.warmup, n_scale)
     late_first = max(1000, start + args.steps)
     late = window_rate(late_first, n_scale)
+    # and the whole transition, from the first steps through nucleation to
+    # the fully broken box (the power limit lowers the clock as it fills)
+    transition = window_rate(args.warmup, 1000)
     log(f"bench: steps {args.warmup}..{args.warmup + n_scale}: {early / 1e9:.2f} G/s; "
-        f"steps {late_first}..{late_first + n_scale}: {late / 1e9:.2f} G/s")
+        f"steps {late_first}..{late_first + n_scale}: {late / 1e9:.2f} G/s; "
+        f"steps {args.warmup}..{args.warmup + 1000}: {transition / 1e9:.2f} G/s")
     value = cells * args.steps / wall_s
     launches = 3 * args.steps
     avg_launch_s = kernel_ms / 1e3 / launches
@@ -648,6 +652,8 @@ def main():
         "stage_kernel_ms_per_launch": avg_launch_s * 1e3,
         "first_steps": {"steps": [args.warmup, args.warmup + n_scale], "site_updates_per_s": early},
         "late_steps": {"steps": [late_first, late_first + n_scale], "site_updates_per_s": late},
+        "transition_steps": {"steps": [args.warmup, args.warmup + 1000], "site_updates_per_s": transition,
+                             "frac": transition * BYTES_PER_SITE_UPDATE / 1e9 / HBM_PEAK_GBS},
         "hbm_context": hbm,
         "batched_sweep": batch,
         "served": served,
