@@ -86,10 +86,10 @@ static hipError_t fused_one(const Box<R> &box, int stage, int zc, hipStream_t st
     return hipGetLastError();
 }
 
-template <int TX, int TY, class M, class R>
+template <int TX, int TY, class M, class R, class E = BagEos>
 static hipError_t fused_multi_one(const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage, hipStream_t st) {
-    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, M, R, true, BagEos>
-                                   : k_stage_fused_multi<TX, TY, M, R, false, BagEos>),
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, M, R, true, E>
+                                   : k_stage_fused_multi<TX, TY, M, R, false, E>),
                        dim3(ntasks), dim3(TX * TY), 0, st, dboxes, devents, dtasks, stage, ntasks);
     return hipGetLastError();
 }
@@ -115,8 +115,15 @@ static hipError_t fused_dispatch(const Box<R> &box, int stage, TileShape t, int 
     }
 }
 template <class M, class R>
-static hipError_t fused_multi_dispatch(const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage, TileShape t,
-                                       hipStream_t st) {
+static hipError_t fused_multi_dispatch(int eos, const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks,
+                                       int stage, TileShape t, hipStream_t st) {
+    if (eos == kEosGammaLaw) {
+        switch (tile_index(t)) {
+        case 2: return fused_multi_one<32, 8, M, R, GammaLawEos>(dboxes, devents, dtasks, ntasks, stage, st);
+        case 3: return fused_multi_one<16, 16, M, R, GammaLawEos>(dboxes, devents, dtasks, ntasks, stage, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (tile_index(t)) {
     case 0: return fused_multi_one<32, 16, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
     case 1: return fused_multi_one<64, 8, M, R>(dboxes, devents, dtasks, ntasks, stage, st);
@@ -140,15 +147,15 @@ hipError_t launch_stage_fused<double>(int, const Box<double> &box, int stage, Ti
     return fused_dispatch<ExactMath<double>, double>(box, stage, t, zc, st);
 }
 template <>
-hipError_t launch_stage_fused_multi<float>(int math, const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks,
-                                           int stage, TileShape t, hipStream_t st) {
-    if (math == BS_MATH_FAST) return launch_stage_fused_multi_fast(dboxes, devents, dtasks, ntasks, stage, t, st);
-    return fused_multi_dispatch<ExactMath<float>, float>(dboxes, devents, dtasks, ntasks, stage, t, st);
+hipError_t launch_stage_fused_multi<float>(int math, int eos, const Box<float> *dboxes, const BoxEvents *devents,
+                                           const Task *dtasks, int ntasks, int stage, TileShape t, hipStream_t st) {
+    if (math == BS_MATH_FAST) return launch_stage_fused_multi_fast(eos, dboxes, devents, dtasks, ntasks, stage, t, st);
+    return fused_multi_dispatch<ExactMath<float>, float>(eos, dboxes, devents, dtasks, ntasks, stage, t, st);
 }
 template <>
-hipError_t launch_stage_fused_multi<double>(int, const Box<double> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
-                                            TileShape t, hipStream_t st) {
-    return fused_multi_dispatch<ExactMath<double>, double>(dboxes, devents, dtasks, ntasks, stage, t, st);
+hipError_t launch_stage_fused_multi<double>(int, int eos, const Box<double> *dboxes, const BoxEvents *devents,
+                                            const Task *dtasks, int ntasks, int stage, TileShape t, hipStream_t st) {
+    return fused_multi_dispatch<ExactMath<double>, double>(eos, dboxes, devents, dtasks, ntasks, stage, t, st);
 }
 
 template <class M, class R, class E>

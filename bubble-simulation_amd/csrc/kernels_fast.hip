@@ -21,10 +21,10 @@ static hipError_t fused_one(const Box<float> &box, int stage, int zc, hipStream_
     return hipGetLastError();
 }
 
-template <int TX, int TY>
+template <int TX, int TY, class E = BagEos>
 static hipError_t fused_multi_one(const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage, hipStream_t st) {
-    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, FastMathF32, float, true, BagEos>
-                                   : k_stage_fused_multi<TX, TY, FastMathF32, float, false, BagEos>),
+    hipLaunchKernelGGL((stage == 0 ? k_stage_fused_multi<TX, TY, FastMathF32, float, true, E>
+                                   : k_stage_fused_multi<TX, TY, FastMathF32, float, false, E>),
                        dim3(ntasks), dim3(TX * TY), 0, st, dboxes, devents, dtasks, stage, ntasks);
     return hipGetLastError();
 }
@@ -45,8 +45,15 @@ hipError_t launch_stage_fused_fast(const Box<float> &box, int stage, TileShape t
     }
 }
 
-hipError_t launch_stage_fused_multi_fast(const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
-                                         TileShape t, hipStream_t st) {
+hipError_t launch_stage_fused_multi_fast(int eos, const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks,
+                                         int ntasks, int stage, TileShape t, hipStream_t st) {
+    if (eos == kEosGammaLaw) {
+        switch (tile_index(t)) {
+        case 2: return fused_multi_one<32, 8, GammaLawEos>(dboxes, devents, dtasks, ntasks, stage, st);
+        case 3: return fused_multi_one<16, 16, GammaLawEos>(dboxes, devents, dtasks, ntasks, stage, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (tile_index(t)) {
     case 0: return fused_multi_one<32, 16>(dboxes, devents, dtasks, ntasks, stage, st);
     case 1: return fused_multi_one<64, 8>(dboxes, devents, dtasks, ntasks, stage, st);

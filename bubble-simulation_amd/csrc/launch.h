@@ -43,18 +43,26 @@ hipError_t launch_reduce(const V2<R> *field, const V4<R> *fluid, const Params<R>
 // one fused RK stage over a whole box; zc = planes per z chunk
 template <class R>
 hipError_t launch_stage_fused(int math, const Box<R> &box, int stage, TileShape t, int zc, hipStream_t st);
-// one fused RK stage over a task table spanning many boxes
+// one fused RK stage over a task table spanning many boxes, all of one
+// equation of state (kEosBag / kEosGammaLaw); the gamma-law tasks of a plan
+// are cut and launched with eos_tile(kEosGammaLaw, plan tile)
 template <class R>
-hipError_t launch_stage_fused_multi(int math, const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
-                                    TileShape t, hipStream_t st);
+hipError_t launch_stage_fused_multi(int math, int eos, const Box<R> *dboxes, const BoxEvents *devents, const Task *dtasks,
+                                    int ntasks, int stage, TileShape t, hipStream_t st);
+// The tile a launch of the given equation of state runs for a requested
+// tile: the bag model is built for every tile; the gamma-law EOS for the
+// two default shapes (16x16, 32x8), other requests run on 32x8.
+constexpr TileShape eos_tile(int eos, TileShape t) {
+    return eos != kEosGammaLaw || (t.tx == 16 && t.ty == 16) || (t.tx == 32 && t.ty == 8) ? t : TileShape{32, 8};
+}
 // primitive pass + stage pass (reference dispatch structure)
 template <class R>
 hipError_t launch_stage_twopass(int math, const Box<R> &box, int stage, V4<R> *prim, hipStream_t st);
 
 // f32 fast-math variants (kernels_fast.hip)
 hipError_t launch_stage_fused_fast(const Box<float> &box, int stage, TileShape t, int zc, hipStream_t st);
-hipError_t launch_stage_fused_multi_fast(const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks, int ntasks, int stage,
-                                         TileShape t, hipStream_t st);
+hipError_t launch_stage_fused_multi_fast(int eos, const Box<float> *dboxes, const BoxEvents *devents, const Task *dtasks,
+                                         int ntasks, int stage, TileShape t, hipStream_t st);
 hipError_t launch_stage_twopass_fast(const Box<float> &box, int stage, V4<float> *prim, hipStream_t st);
 
 // batch bookkeeping kernels (kernels_exact.hip)

@@ -14,11 +14,11 @@ int Entry::create(const bs_spec &s, hipStream_t stream, bool async) {
     std::string err;
     int rc = spec_validate(s, err);
     if (rc) return set_error(rc, err);
-    if (s.eos != BS_EOS_BAG) return set_error(BS_ERR_INVALID, "the batched engine runs the bag equation of state only");
     rc = spec_schedule(s, schedule, err);
     if (rc) return set_error(rc, err);
     rc = bx.alloc(s.precision, s.grid, stream, async);
     if (rc) return rc;
+    bx.set_eos(s);
     sim_params_fill(s.model, s.grid, 0.0f, &bx.params);
     bx.cur = 0;
     rc = bx.init(stream);
@@ -81,7 +81,8 @@ Engine::~Engine() {
     if (d_events) (void)hipFree(d_events);
     if (d_flags) (void)hipFree(d_flags);
     for (Plan &p : plan)
-        if (p.d_tasks) (void)hipFree(p.d_tasks);
+        for (Task *t : p.d_tasks)
+            if (t) (void)hipFree(t);
     for (int i = 0; i < 2; ++i) {
         if (stage_buf[i]) (void)hipHostFree(stage_buf[i]);
         if (stage_done[i]) (void)hipEventDestroy(stage_done[i]);
@@ -130,13 +131,14 @@ template <class R> static void fill_boxes(const std::vector<Entry *> &es, unsign
     for (size_t i = 0; i < es.size(); ++i) b[i] = es[i]->bx.box<R>();
 }
 
-// tasks of one stage class: (box, x0, y0, zb, ze) over every box, padded
-// with empty tasks to a multiple of 8, so that the kernel can hand each XCD
-// a contiguous run of tiles (blocks go round-robin over the 8 XCDs;
-// neighbouring tiles then share that XCD's L2)
-static void build_tasks(const std::vector<Entry *> &es, TileShape tile, int zc, std::vector<Task> &tasks) {
+// tasks of one stage class and equation of state: (box, x0, y0, zb, ze)
+// over every box of that EOS, padded with empty tasks to a multiple of 8,
+// so that the kernel can hand each XCD a contiguous run of tiles (blocks go
+// round-robin over the 8 XCDs; neighbouring tiles then share that XCD's L2)
+static void build_tasks(const std::vector<Entry *> &es, TileShape tile, int zc, int eos, std::vector<Task> &tasks) {
     tasks.clear();
     for (size_t i = 0; i < es.size(); ++i) {
+        if (es[i]->spec.eos != eos) continue;
         const uint32_t *g = es[i]->spec.grid;
         for (uint32_t zb = 0; zb < g[2]; zb += (uint32_t)zc)
             for (uint32_t y0 = 0; y0 < g[1]; y0 += (uint32_t)tile.ty)
@@ -162,15 +164,17 @@ int Engine::upload() {
     drop_graph();
     const size_t n = entries.size();
     const size_t box_bytes = precision == BS_PRECISION_F64 ? sizeof(Box<double>) : sizeof(Box<float>);
-    std::vector<Task> tasks[2];
-    for (int p = 0; p < 2; ++p) {
-        build_tasks(entries, plan[p].tile, plan[p].zc, tasks[p]);
-        plan[p].ntasks = (int)tasks[p].size();
-    }
+    std::vector<Task> tasks[2][kEosClasses];
+    bool grow_tasks = false;
+    for (int p = 0; p < 2; ++p)
+        for (int c = 0; c < kEosClasses; ++c) {
+            build_tasks(entries, eos_tile(c, plan[p].tile), plan[p].zc, c, tasks[p][c]);
+            plan[p].ntasks[c] = (int)tasks[p][c].size();
+            grow_tasks = grow_tasks || (size_t)plan[p].ntasks[c] > plan[p].cap[c];
+        }
     // device arrays grow rarely; a launch still queued may read the old
     // ones, so the stream is drained before they are freed
     const bool grow_boxes = n > cap_boxes;
-    const bool grow_tasks = (size_t)plan[0].ntasks > plan[0].cap || (size_t)plan[1].ntasks > plan[1].cap;
     if (grow_boxes || grow_tasks) BS_HIP_TRY(hipStreamSynchronize(stream));
     if (grow_boxes) {
         if (d_boxes) (void)hipFree(d_boxes);
@@ -187,20 +191,25 @@ int Engine::upload() {
         BS_HIP_TRY(hipMemsetAsync(d_flags, 0xff, cap * sizeof(uint32_t), stream));  // no tick matches
         cap_boxes = cap;
     }
-    for (Plan &p : plan) {
-        if ((size_t)p.ntasks <= p.cap) continue;
-        if (p.d_tasks) (void)hipFree(p.d_tasks);
-        p.d_tasks = nullptr;
-        p.cap = 0;
-        const size_t cap = std::max<size_t>((size_t)p.ntasks, 256);
-        BS_HIP_TRY(hipMalloc((void **)&p.d_tasks, cap * sizeof(Task)));
-        p.cap = cap;
-    }
-    // stage everything in pinned memory: [boxes | events | tasks 0 | tasks 1]
+    for (Plan &p : plan)
+        for (int c = 0; c < kEosClasses; ++c) {
+            if ((size_t)p.ntasks[c] <= p.cap[c]) continue;
+            if (p.d_tasks[c]) (void)hipFree(p.d_tasks[c]);
+            p.d_tasks[c] = nullptr;
+            p.cap[c] = 0;
+            const size_t cap = std::max<size_t>((size_t)p.ntasks[c], 256);
+            BS_HIP_TRY(hipMalloc((void **)&p.d_tasks[c], cap * sizeof(Task)));
+            p.cap[c] = cap;
+        }
+    // stage everything in pinned memory: [boxes | events | the task tables]
     const size_t off_ev = (n * box_bytes + 255) & ~(size_t)255;
-    const size_t off_t0 = (off_ev + n * sizeof(BoxEvents) + 255) & ~(size_t)255;
-    const size_t off_t1 = (off_t0 + tasks[0].size() * sizeof(Task) + 255) & ~(size_t)255;
-    const size_t total = off_t1 + tasks[1].size() * sizeof(Task);
+    size_t off_t[2][kEosClasses];
+    size_t total = (off_ev + n * sizeof(BoxEvents) + 255) & ~(size_t)255;
+    for (int p = 0; p < 2; ++p)
+        for (int c = 0; c < kEosClasses; ++c) {
+            off_t[p][c] = total;
+            total = (total + tasks[p][c].size() * sizeof(Task) + 255) & ~(size_t)255;
+        }
     const int k = stage_next;
     stage_next ^= 1;
     if (!stage_done[k]) BS_HIP_TRY(hipEventCreateWithFlags(&stage_done[k], hipEventDisableTiming));
@@ -228,18 +237,18 @@ int Engine::upload() {
         ev[i].steps_target = e->steps_target;
         ev[i].active = 1;
     }
-    if (!tasks[0].empty()) std::memcpy(buf + off_t0, tasks[0].data(), tasks[0].size() * sizeof(Task));
-    if (!tasks[1].empty()) std::memcpy(buf + off_t1, tasks[1].data(), tasks[1].size() * sizeof(Task));
     if (n) {
         BS_HIP_TRY(hipMemcpyAsync(d_boxes, buf, n * box_bytes, hipMemcpyHostToDevice, stream));
         BS_HIP_TRY(hipMemcpyAsync(d_events, buf + off_ev, n * sizeof(BoxEvents), hipMemcpyHostToDevice, stream));
     }
-    if (!tasks[0].empty())
-        BS_HIP_TRY(hipMemcpyAsync(plan[0].d_tasks, buf + off_t0, tasks[0].size() * sizeof(Task),
-                                  hipMemcpyHostToDevice, stream));
-    if (!tasks[1].empty())
-        BS_HIP_TRY(hipMemcpyAsync(plan[1].d_tasks, buf + off_t1, tasks[1].size() * sizeof(Task),
-                                  hipMemcpyHostToDevice, stream));
+    for (int p = 0; p < 2; ++p)
+        for (int c = 0; c < kEosClasses; ++c) {
+            const std::vector<Task> &t = tasks[p][c];
+            if (t.empty()) continue;
+            std::memcpy(buf + off_t[p][c], t.data(), t.size() * sizeof(Task));
+            BS_HIP_TRY(hipMemcpyAsync(plan[p].d_tasks[c], buf + off_t[p][c], t.size() * sizeof(Task),
+                                      hipMemcpyHostToDevice, stream));
+        }
     BS_HIP_TRY(hipEventRecord(stage_done[k], stream));
     return BS_OK;
 }
@@ -264,14 +273,23 @@ int Engine::download_progress() {
     return BS_OK;
 }
 
+// one stage over every task of the set: a launch per equation of state present
+template <class R> static hipError_t launch_stage_all(const Engine &e, int s) {
+    const Plan &p = e.plan[s == 0 ? 0 : 1];
+    for (int c = 0; c < kEosClasses; ++c) {
+        if (!p.ntasks[c]) continue;
+        const hipError_t r = launch_stage_fused_multi<R>(e.math, c, (const Box<R> *)e.d_boxes, e.d_events, p.d_tasks[c],
+                                                         p.ntasks[c], s, eos_tile(c, p.tile), e.stream);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
 template <class R> static int step_eager_t(Engine &e) {
     const int n = (int)e.entries.size();
     Box<R> *db = (Box<R> *)e.d_boxes;
     BS_HIP_TRY(launch_stamp_multi<R>(db, e.d_events, n, e.stream));
-    for (int s = 0; s < 3; ++s) {
-        const Plan &p = e.plan[s == 0 ? 0 : 1];
-        BS_HIP_TRY(launch_stage_fused_multi<R>(e.math, db, e.d_events, p.d_tasks, p.ntasks, s, p.tile, e.stream));
-    }
+    for (int s = 0; s < 3; ++s) BS_HIP_TRY(launch_stage_all<R>(e, s));
     BS_HIP_TRY(launch_advance_multi<R>(db, e.d_events, n, e.stream));
     return BS_OK;
 }
@@ -336,12 +354,7 @@ int Engine::time_stages(float ms[2]) {
     // stage 1 reads slot cur and writes the next slot, stages 2 and 3 read
     // those and write the slots after cur: repeating them leaves U^n untouched
     auto launch = [&](int s) -> hipError_t {
-        const Plan &p = plan[s == 0 ? 0 : 1];
-        if (precision == BS_PRECISION_F64)
-            return launch_stage_fused_multi<double>(math, (Box<double> *)d_boxes, d_events, p.d_tasks, p.ntasks, s,
-                                                    p.tile, stream);
-        return launch_stage_fused_multi<float>(math, (Box<float> *)d_boxes, d_events, p.d_tasks, p.ntasks, s, p.tile,
-                                               stream);
+        return precision == BS_PRECISION_F64 ? launch_stage_all<double>(*this, s) : launch_stage_all<float>(*this, s);
     };
     constexpr int kReps = 3;
     for (int s = 0; s < 3 && e == hipSuccess; ++s) e = launch(s);  // warm
@@ -394,7 +407,7 @@ int autotune(Engine &eng) {
     std::string key = std::to_string(eng.precision) + ":" + std::to_string(eng.math);
     for (Entry *e : eng.entries)
         key += ":" + std::to_string(e->spec.grid[0]) + "x" + std::to_string(e->spec.grid[1]) + "x" +
-               std::to_string(e->spec.grid[2]);
+               std::to_string(e->spec.grid[2]) + "e" + std::to_string(e->spec.eos);
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);

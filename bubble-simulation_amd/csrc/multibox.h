@@ -36,15 +36,22 @@ struct Entry {
     void host_advance(uint64_t n);
 };
 
+// Equation-of-state classes of a set (BS_EOS_BAG, BS_EOS_GAMMA_LAW): each
+// stage launch runs the tasks of one class, so a set that holds both takes
+// two launches per stage and a bag-only set runs the bag kernels alone.
+constexpr int kEosClasses = 2;
+
 // Tile shape and z chunk of one class of stage launches, with its task
-// table: the first RK stage (reads U_in only) and the other two (stream U^n
-// as well) are bound differently and may take different tiles.
+// tables (one per equation of state): the first RK stage (reads U_in only)
+// and the other two (stream U^n as well) are bound differently and may take
+// different tiles.  The gamma-law tasks are cut with eos_tile(kEosGammaLaw,
+// tile).
 struct Plan {
     TileShape tile{16, 16};
     int zc = 32;
-    Task *d_tasks = nullptr;
-    int ntasks = 0;
-    size_t cap = 0;
+    Task *d_tasks[kEosClasses] = {nullptr, nullptr};
+    int ntasks[kEosClasses] = {0, 0};
+    size_t cap[kEosClasses] = {0, 0};
 };
 
 // The launch machinery over a set of entries.

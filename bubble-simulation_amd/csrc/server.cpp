@@ -433,10 +433,6 @@ int bs_server_submit(bs_server *s, const bs_job *job, uint64_t *job_id_out) {
     sp.kernel = BS_KERNEL_FUSED;
     std::string err;
     int rc = spec_validate(sp, err);
-    if (!rc && sp.eos != BS_EOS_BAG) {
-        err = "the server runs the bag equation of state only";
-        rc = BS_ERR_INVALID;
-    }
     if (rc) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.rejected += 1;

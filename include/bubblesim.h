@@ -91,8 +91,8 @@ enum { BS_NUC_SIMULTANEOUS = 0, BS_NUC_EXPONENTIAL = 1, BS_NUC_UNIFORM_STEPS = 2
  *                     eos_gamma = 4/3 is the bag model.
  * dof_ratio (docs/EOS.md:65-84) interpolates the radiation constant of the
  * temperature across the wall, a_broken / a_symmetric; it changes only the
- * temperature channel of the visualisation.  The batched engine and the
- * server run the bag model. */
+ * temperature channel of the visualisation.  A batch or a server may mix
+ * both models (one stage launch per model present). */
 enum { BS_EOS_BAG = 0, BS_EOS_GAMMA_LAW = 1 };
 
 /* Model (physics.rs:44-68).  seed_kind/seed_value encode SeedSize. */

@@ -5,7 +5,9 @@
 * FastMath gamma-law stays within twice the exact f32 path's distance to f64;
 * a box at rest stays exactly at rest; Gamma = 4/3 tracks the bag model;
 * the dof-ratio temperature channel equals the oracle's, bit for bit in half;
-* the batched engine and the server refuse a non-bag EOS.
+* a batch or a server may mix bag and gamma-law boxes: each box gives the
+  bits it gives alone (ExactMath against a plain Simulation, graph replay
+  against eager stepping under FastMath).
 """
 import numpy as np
 import pytest
@@ -117,13 +119,85 @@ def test_dof_ratio_texels_equal_oracle(bs, orc, gpu_available):
     o.close()
 
 
-def test_batch_and_server_refuse_other_eos(bs, gpu_available):
-    spec = bs.make_spec(grid=16, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, eos=bs.EOS_GAMMA_LAW, eos_gamma=1.5)
-    with pytest.raises(bs.BubbleSimError) as e:
-        bs.Batch([spec])
-    assert e.value.code == bs.BS_ERR_INVALID
-    srv = bs.Server(budget_bytes=1 << 30, chunk_steps=5)
-    with pytest.raises(bs.BubbleSimError) as e:
-        srv.submit(spec, 10)
-    assert e.value.code == bs.BS_ERR_INVALID
+def _mixed_specs(bs, math, precision=0, n=6, seed=21):
+    # bag and gamma-law boxes (several Gamma) of uneven size in one set
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n):
+        grid = (int(rng.choice([16, 24, 40])), int(rng.choice([16, 24])), int(rng.choice([16, 20])))
+        model = bs.Model(eta=float(rng.uniform(0.05, 0.5)), seed_factor=float(rng.uniform(0.9, 1.4)))
+        gamma = (1.45, 1.3, 1.6)[k % 3]
+        eos = bs.EOS_GAMMA_LAW if k % 2 == 0 else bs.EOS_BAG
+        specs.append(bs.make_spec(grid=grid, model=model, nucleation=bs.NUC_UNIFORM_STEPS, bubbles=int(rng.integers(1, 4)),
+                                  seed=k + 1, uniform_max_step=8, time_seed=300 + k, math=math, precision=precision,
+                                  eos=eos, eos_gamma=gamma if eos == bs.EOS_GAMMA_LAW else 4.0 / 3.0))
+    return specs
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("tile", [None, "32x16"])
+def test_batch_mixed_eos_equals_single_boxes_exact(bs, gpu_available, monkeypatch, precision, tile):
+    # a batch holding bag and gamma-law boxes (one stage launch per model):
+    # every box bit for bit as alone, eager and through graph replay; with a
+    # plan tile the gamma-law kernel is not built for (32x16), its tasks run
+    # on 32x8
+    if tile:
+        monkeypatch.setenv("BS_TILE", tile)
+    print(f"case precision={precision} tile={tile}", flush=True)
+    specs = _mixed_specs(bs, bs.MATH_EXACT, precision)
+    b = bs.Batch(specs)
+    b.advance(6, use_graph=False)
+    b.advance(30, use_graph=True)
+    assert b.graph_steps() == 25
+    for i, s in enumerate(specs):
+        sim = bs.Simulation(s)
+        sim.advance(36)
+        sf, su = sim.download()
+        bf, bu = b.download(i)
+        assert np.array_equal(sf, bf) and np.array_equal(su, bu), f"box {i} (eos {s.eos}) differs from its solo run"
+        sim.close()
+    b.close()
+
+
+def test_batch_mixed_eos_graph_equals_eager_fast(bs, gpu_available):
+    specs = _mixed_specs(bs, bs.MATH_FAST)
+    g = bs.Batch(specs)
+    e = bs.Batch(specs)
+    g.advance(50, use_graph=True)
+    e.advance(50, use_graph=False)
+    assert g.graph_steps() == 50 and e.graph_steps() == 0
+    for i in range(len(specs)):
+        gf, gu = g.download(i)
+        ef, eu = e.download(i)
+        assert np.array_equal(gf, ef) and np.array_equal(gu, eu), f"box {i}"
+        assert g.diagnostics(i).nonfinite == 0
+    g.close()
+    e.close()
+
+
+def test_server_mixed_eos(bs, gpu_available):
+    # gamma-law and bag jobs in one server: the same bits alone and in the mix;
+    # under ExactMath a served gamma-law job equals a plain Simulation
+    specs = _mixed_specs(bs, bs.MATH_FAST, seed=22)
+    steps = [30, 45, 20, 60, 35, 50]
+    srv = bs.Server(budget_bytes=2 << 30, chunk_steps=5)
+    alone = [srv.wait(srv.submit(s, n, checksum=True), 300) for s, n in zip(specs, steps)]
+    assert all(r.status == 0 for r in alone)
+    ids = [srv.submit(s, n, checksum=True) for s, n in zip(specs, steps)]
+    mixed = [srv.wait(j, 300) for j in ids]
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert m.status == 0 and m.steps == steps[k]
+        assert m.state_checksum == a.state_checksum, f"job {k} (eos {specs[k].eos}) differs in the mix"
+    srv.close()
+    exact = _mixed_specs(bs, bs.MATH_EXACT, seed=22)[:2]
+    srv = bs.Server(budget_bytes=2 << 30, chunk_steps=4, math=bs.MATH_EXACT)
+    ids = [srv.submit(s, 13, keep_state=True) for s in exact]
+    for s, j in zip(exact, ids):
+        assert srv.wait(j, 300).status == 0
+        f, u = srv.fetch_state(j, s)
+        sim = bs.Simulation(s)
+        sim.advance(13)
+        sf, su = sim.download()
+        assert np.array_equal(f, sf) and np.array_equal(u, su)
+        sim.close()
     srv.close()
