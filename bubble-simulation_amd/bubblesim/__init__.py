@@ -130,6 +130,21 @@ class CServerStats(C.Structure):
                 ("failed", C.c_uint64), ("nonfinite", C.c_uint64)]
 
 
+class CRenderSettings(C.Structure):
+    _fields_ = [("field_mode", C.c_int32), ("colormap", C.c_int32), ("iso_level", C.c_float),
+                ("wall_opacity", C.c_float), ("samples_per_cell", C.c_float), ("exposure", C.c_float),
+                ("field_gain", C.c_float), ("absorption", C.c_float), ("show_iso", C.c_int32),
+                ("show_volume", C.c_int32), ("show_box", C.c_int32), ("background", C.c_float),
+                ("clip_axis", C.c_int32), ("clip_pos", C.c_float)]
+
+
+class CRenderView(C.Structure):
+    _fields_ = [("inv_view_proj", C.c_float * 16)] + [
+        (n, C.c_float * 4) for n in ("cam_pos", "box_half", "grid", "render0", "render1", "render2", "render3")]
+
+
+FIELD_PHI, FIELD_SPEED, FIELD_TEMPERATURE, FIELD_KINETIC = range(4)
+
 _P = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
 _SIGS = {
@@ -185,6 +200,12 @@ _SIGS = {
     "bs_sim_record_visualisation": (C.c_int, [_P]),
     "bs_sim_visualisation_buffer": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint64)]),
     "bs_sim_download_visualisation": (C.c_int, [_P, C.POINTER(C.c_uint16), C.c_uint64]),
+    "bs_render_settings_default": (C.c_int, [C.c_int32, C.POINTER(CRenderSettings)]),
+    "bs_render_view_make": (C.c_int, [_u32p, C.POINTER(CRenderSettings), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.POINTER(CRenderView)]),
+    "bs_sim_render": (C.c_int, [_P, C.POINTER(CRenderView), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]),
+    "bs_sim_screenshot": (C.c_int, [_P, C.POINTER(CRenderSettings), C.c_uint32, C.c_uint32, C.c_char_p]),
+    "bs_png_write": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32]),
     "bs_sim_download": (C.c_int, [_P, _P, _P]),
     "bs_sim_upload": (C.c_int, [_P, _P, _P]),
     "bs_sim_advance_timed": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -442,6 +463,30 @@ def diag_dict(d: CDiagnostics) -> dict:
 # ---------------------------------------------------------------------------
 #  Simulation (sim.rs:180-1026)
 # ---------------------------------------------------------------------------
+def render_settings(field_mode: int = FIELD_KINETIC) -> CRenderSettings:
+    """RenderSettings::with_field (render.rs:96-105)."""
+    st = CRenderSettings()
+    _check(lib().bs_render_settings_default(field_mode, C.byref(st)))
+    return st
+
+
+def render_view(grid, settings: Optional[CRenderSettings] = None, width: int = 1280, height: int = 960,
+                frame: int = 1) -> CRenderView:
+    """The view uniform of the default orbit camera framed on the box (render.rs:288-334)."""
+    v = CRenderView()
+    settings = settings if settings is not None else render_settings()
+    _check(lib().bs_render_view_make(_grid3(grid), C.byref(settings), width, height, frame, C.byref(v)))
+    return v
+
+
+def png_write(path: str, rgba: np.ndarray):
+    """The screenshot's PNG writer: rgba of shape (height, width, 4), uint8."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w, c = rgba.shape
+    assert c == 4
+    _check(lib().bs_png_write(os.fsencode(path), rgba.ctypes.data_as(C.POINTER(C.c_uint8)), w, h))
+
+
 class Simulation:
     """One lattice box on the GPU (Simulation, sim.rs:180)."""
 
@@ -579,6 +624,18 @@ class Simulation:
         out = np.zeros((nz, ny, nx, 4), dtype=np.uint16)
         _check(lib().bs_sim_download_visualisation(self._h, out.ctypes.data_as(C.POINTER(C.c_uint16)), out.size))
         return out.view(np.float16)
+
+    # offscreen volume rendering (volume.wgsl, capture.rs)
+    def render(self, view: "CRenderView", width: int, height: int) -> np.ndarray:
+        """RGBA8 pixels of the last recorded visualisation, shape (height, width, 4), row 0 at the top."""
+        out = np.zeros((height, width, 4), dtype=np.uint8)
+        _check(lib().bs_sim_render(self._h, C.byref(view), width, height, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def screenshot(self, path: str, settings: Optional["CRenderSettings"] = None, size=(1280, 960)):
+        """capture::render_to_png: record the visualisation, render frame 1, write an RGBA PNG."""
+        settings = settings if settings is not None else render_settings()
+        _check(lib().bs_sim_screenshot(self._h, C.byref(settings), size[0], size[1], os.fsencode(path)))
 
     # state (debug dump / load)
     def download(self):

@@ -47,6 +47,8 @@ size_t bs_struct_size(int which) {
     case 11: return sizeof(bs_job);
     case 12: return sizeof(bs_job_result);
     case 13: return sizeof(bs_server_stats);
+    case 14: return sizeof(bs_render_settings);
+    case 15: return sizeof(bs_render_view);
     default: return 0;
     }
 }

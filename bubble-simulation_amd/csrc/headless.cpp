@@ -49,12 +49,12 @@ void usage(FILE *f) {
                  "      --nucleation-duration <T>      Time over which exponential nucleation completes [default: 60]\n"
                  "      --seed <SEED>                  Seed for bubble placement [default: 1]\n"
                  "      --steps-per-frame <N>          (interactive only) [default: 2]\n"
-                 "      --field <FIELD>                (interactive only) [default: kinetic]\n"
+                 "      --field <FIELD>                Field the volume renderer shows: phi | speed | temperature | kinetic [default: kinetic]\n"
                  "      --no-vsync                     (interactive only)\n"
                  "      --headless <STEPS>             Run this many steps without a window, printing diagnostics\n"
                  "      --report-every <N>             Report interval in headless mode [default: steps/20]\n"
-                 "      --screenshot <PATH>            (rendering is not part of this build)\n"
-                 "      --screenshot-size <WxH>        [default: 1280x960]\n"
+                 "      --screenshot <PATH>            Render one frame to this PNG and exit (after --headless N steps)\n"
+                 "      --screenshot-size <WxH>        Resolution of --screenshot [default: 1280x960]\n"
                  "  extensions:\n"
                  "      --precision <f32|f64>          Lattice precision [default: f32]\n"
                  "      --math <fast|exact>            Arithmetic policy of the stage kernels [default: fast]\n"
@@ -81,6 +81,42 @@ bool parse_u64(const std::string &v, uint64_t &out) {
     return true;
 }
 
+// parse_size (config.rs:181-192): WIDTHxHEIGHT, either case of x, both
+// positive
+bool parse_size(const std::string &s, uint32_t &w, uint32_t &h, std::string &err) {
+    const size_t k = s.find_first_of("xX");
+    if (k == std::string::npos) {
+        err = "`" + s + "` is not a size; expected WIDTHxHEIGHT";
+        return false;
+    }
+    auto trim = [](std::string v) {
+        const size_t a = v.find_first_not_of(" \t"), b = v.find_last_not_of(" \t");
+        return a == std::string::npos ? std::string() : v.substr(a, b - a + 1);
+    };
+    const std::string parts[2] = {trim(s.substr(0, k)), trim(s.substr(k + 1))};
+    const char *what[2] = {"width", "height"};
+    uint32_t out[2];
+    for (int i = 0; i < 2; ++i) {
+        uint64_t u = 0;
+        if (parts[i].empty()) {
+            err = std::string(what[i]) + ": cannot parse integer from empty string";
+            return false;
+        }
+        if (!parse_u64(parts[i], u) || u > 0xffffffffULL) {
+            err = std::string(what[i]) + ": invalid digit found in string";
+            return false;
+        }
+        if (u == 0) {
+            err = std::string(what[i]) + " must be positive";
+            return false;
+        }
+        out[i] = (uint32_t)u;
+    }
+    w = out[0];
+    h = out[1];
+    return true;
+}
+
 int fail_usage(const std::string &msg) {
     std::fprintf(stderr, "error: %s\n\nFor more information, try '--help'.\n", msg.c_str());
     return 2;
@@ -99,6 +135,9 @@ int main(int argc, char **argv) {
     bool have_report = false;
     uint64_t steps_per_frame = 2;
     std::string dump_path;
+    std::string screenshot_path;
+    uint32_t shot_w = 1280, shot_h = 960;  // config.rs:112-114
+    int32_t field_mode = 3;                // FieldMode::Kinetic (render.rs:26-28)
 
     std::vector<std::string> args(argv + 1, argv + argc);
     for (size_t i = 0; i < args.size(); ++i) {
@@ -168,11 +207,18 @@ int main(int argc, char **argv) {
             else if (v == "uniform-steps") spec.nucleation = BS_NUC_UNIFORM_STEPS;
             else return fail_usage("invalid value '" + v + "' for '--nucleation <NUCLEATION>'");
         } else if (a == "--field") {
-            if (v != "phi" && v != "speed" && v != "temperature" && v != "kinetic")
-                return fail_usage("invalid value '" + v + "' for '--field <FIELD>'");
+            if (v == "phi") field_mode = 0;
+            else if (v == "speed") field_mode = 1;
+            else if (v == "temperature") field_mode = 2;
+            else if (v == "kinetic") field_mode = 3;
+            else return fail_usage("invalid value '" + v + "' for '--field <FIELD>'");
         } else if (a == "--screenshot") {
             have_screenshot = true;
+            screenshot_path = v;
         } else if (a == "--screenshot-size") {
+            std::string err;
+            if (!parse_size(v, shot_w, shot_h, err))
+                return fail_usage("invalid value '" + v + "' for '--screenshot-size <SCREENSHOT_SIZE>': " + err);
         } else if (a == "--precision") {
             if (v == "f32") spec.precision = BS_PRECISION_F32;
             else if (v == "f64") spec.precision = BS_PRECISION_F64;
@@ -208,9 +254,10 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "Error: --steps-per-frame must be at least 1\n");
         return 1;
     }
-    if (have_screenshot) {
-        std::fprintf(stderr, "Error: --screenshot needs the volume renderer, which is not part of this build\n");
-        return 1;
+    // a screenshot implies a windowless run (main.rs:29-33)
+    if (!have_headless && have_screenshot) {
+        have_headless = true;
+        headless = 0;
     }
     if (!have_headless) {
         std::fprintf(stderr, "Error: the interactive window is not part of this build; run with --headless <STEPS>\n");
@@ -307,6 +354,17 @@ int main(int argc, char **argv) {
                 rc = 1;
             }
             if (f) std::fclose(f);
+        }
+    }
+    if (!rc && have_screenshot) {
+        // capture::render_to_png with RenderSettings::with_field (headless.rs:102-113)
+        bs_render_settings rs;
+        if (bs_render_settings_default(field_mode, &rs) != BS_OK ||
+            bs_sim_screenshot(sim, &rs, shot_w, shot_h, screenshot_path.c_str()) != BS_OK) {
+            std::fprintf(stderr, "Error: %s\n", bs_last_error());
+            rc = 1;
+        } else {
+            std::printf("wrote %s (%ux%u)\n", screenshot_path.c_str(), shot_w, shot_h);
         }
     }
     bs_sim_destroy(sim);

@@ -69,6 +69,10 @@ hipError_t launch_stage_twopass_fast(const Box<float> &box, int stage, V4<float>
 template <class R>
 hipError_t launch_stamp_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
 template <class R> hipError_t launch_advance_multi(Box<R> *dboxes, BoxEvents *devents, int nboxes, hipStream_t st);
+// volume.wgsl's ray march over rgba16f texels (render.hip): width x height
+// RGBA8 pixels, row 0 at the top, into device memory
+hipError_t launch_render_volume(const void *texels, const uint32_t grid[3], const bs_render_view &view,
+                                uint32_t width, uint32_t height, void *rgba_out, hipStream_t st);
 // flags[box] = tick for every box whose slot cur holds a non-finite value
 template <class R>
 hipError_t launch_finite_multi(const Box<R> *dboxes, int nboxes, uint32_t *flags, uint32_t tick, hipStream_t st);

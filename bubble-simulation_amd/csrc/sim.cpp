@@ -404,6 +404,39 @@ int bs_sim_download_visualisation(bs_sim *s, uint16_t *out, uint64_t n_halfs) {
     return BS_OK;
 }
 
+// volume.wgsl fs_main over the recorded texels (render.hip)
+int bs_sim_render(bs_sim *s, const bs_render_view *view, uint32_t width, uint32_t height, uint8_t *rgba_out) {
+    if (!s || !view || !rgba_out) return set_error(BS_ERR_INVALID, "null argument");
+    if (!s->vis) return set_error(BS_ERR_NOT_READY, "no visualisation recorded yet");
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return set_error(BS_ERR_INVALID, "the frame must be 1x1 to 16384x16384 pixels");
+    const size_t bytes = (size_t)width * height * 4;
+    void *d_out = nullptr;
+    BS_HIP_TRY(hipMalloc(&d_out, bytes));
+    hipError_t e = launch_render_volume(s->vis, s->bx.grid, *view, width, height, d_out, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba_out, d_out, bytes, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return set_error(BS_ERR_DEVICE, std::string("volume render: ") + hipGetErrorString(e));
+    return BS_OK;
+}
+
+// capture::render_to_png (capture.rs:19-117): the visualisation of the
+// current state, frame 1 of a fresh renderer, to an RGBA PNG
+int bs_sim_screenshot(bs_sim *s, const bs_render_settings *settings, uint32_t width, uint32_t height,
+                      const char *path) {
+    if (!s || !settings || !path) return set_error(BS_ERR_INVALID, "null argument");
+    bs_render_view view;
+    int rc = bs_render_view_make(s->bx.grid, settings, width, height, 1, &view);
+    if (rc) return rc;
+    rc = bs_sim_record_visualisation(s);
+    if (rc) return rc;
+    std::vector<uint8_t> px((size_t)width * height * 4);
+    rc = bs_sim_render(s, &view, width, height, px.data());
+    if (rc) return rc;
+    return bs_png_write(path, px.data(), width, height);
+}
+
 int bs_sim_download(bs_sim *s, void *field, void *fluid) {
     if (!s) return set_error(BS_ERR_INVALID, "null simulation");
     return download_state(s->bx, s->stream, field, fluid);

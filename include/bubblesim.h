@@ -379,6 +379,70 @@ int bs_sim_record_visualisation(bs_sim *sim);
 int bs_sim_visualisation_buffer(bs_sim *sim, void **device_ptr, uint64_t *bytes);
 /* Blocking copy of the texels: 4 halves per cell (n_halfs >= 4 * cells). */
 int bs_sim_download_visualisation(bs_sim *sim, uint16_t *out, uint64_t n_halfs);
+
+/* ---- offscreen volume rendering (--screenshot) ----------------------------
+ * The reference draws the texels with a ray-marching fragment shader
+ * (volume.wgsl, gpu/render.rs) and writes one offscreen frame to a PNG
+ * (capture.rs:19-117).  Here the same ray march is a HIP kernel, one thread
+ * per pixel, with the trilinear, clamp-to-edge texel filter written out. */
+
+/* RenderSettings (render.rs:73-126); field_mode: 0 phi, 1 speed,
+ * 2 temperature, 3 kinetic (FieldMode); colormap: 0 viridis, 1 inferno,
+ * 2 signed; clip_axis -1 disables the cutaway. */
+typedef struct bs_render_settings {
+    int32_t field_mode;
+    int32_t colormap;
+    float iso_level;
+    float wall_opacity;
+    float samples_per_cell;
+    float exposure;
+    float field_gain;
+    float absorption;
+    int32_t show_iso;
+    int32_t show_volume;
+    int32_t show_box;
+    float background;
+    int32_t clip_axis;
+    float clip_pos;
+} bs_render_settings;
+
+/* The view uniform of volume.wgsl:22-31 / ViewUniform (render.rs:128-139):
+ * inverse view-projection (column-major), eye, box half-extents with the
+ * cell size in w, grid, and the packed settings; 176 bytes. */
+typedef struct bs_render_view {
+    float inv_view_proj[16];
+    float cam_pos[4];
+    float box_half[4];
+    float grid[4];
+    float render0[4]; /* iso_level, wall_opacity, samples_per_cell, exposure */
+    float render1[4]; /* field_mode, clip_axis, clip_pos, colormap */
+    float render2[4]; /* field_gain, absorption, show_iso, show_volume */
+    float render3[4]; /* frame jitter, show_box, background, unused */
+} bs_render_view;
+
+/* RenderSettings::with_field (render.rs:96-105): the defaults with the
+ * field's gain and colour map (FieldMode::default_gain/default_colormap). */
+int bs_render_settings_default(int32_t field_mode, bs_render_settings *out);
+/* The uniform VolumeRenderer::update writes (render.rs:288-334) for the
+ * default orbit camera framed on the box (camera.rs:24-34, frame_box
+ * camera.rs:98-101; box_dimensions render.rs:363-371), a width x height
+ * target, and frame number `frame` (the dither offset is (frame % 64) *
+ * 0.618034; capture.rs renders frame 1). */
+int bs_render_view_make(const uint32_t grid[3], const bs_render_settings *settings, uint32_t width, uint32_t height,
+                        uint32_t frame, bs_render_view *out);
+/* volume.wgsl fs_main over the texels of the last
+ * bs_sim_record_visualisation: writes width x height RGBA8 pixels, row 0 at
+ * the top, to host memory (blocking).  BS_ERR_NOT_READY if nothing was
+ * recorded yet. */
+int bs_sim_render(bs_sim *sim, const bs_render_view *view, uint32_t width, uint32_t height, uint8_t *rgba_out);
+/* capture::render_to_png (capture.rs:19-117): records the visualisation of
+ * the current state, renders frame 1 with `settings` at width x height and
+ * writes an 8-bit RGBA PNG to `path`. */
+int bs_sim_screenshot(bs_sim *sim, const bs_render_settings *settings, uint32_t width, uint32_t height,
+                      const char *path);
+/* The PNG writer of bs_sim_screenshot: 8-bit RGBA, rows top to bottom,
+ * zlib-compressed (no GPU involved). */
+int bs_png_write(const char *path, const uint8_t *rgba, uint32_t width, uint32_t height);
 /* Debug dump / load of U^n (slot `cur`), layout in the header comment. */
 int bs_sim_download(bs_sim *sim, void *field, void *fluid);
 int bs_sim_upload(bs_sim *sim, const void *field, const void *fluid);

@@ -65,6 +65,9 @@ def lib():
         L.oracle_tanh_f64.argtypes = [C.c_double]
         L.oracle_tanh_f32.restype = C.c_float
         L.oracle_tanh_f32.argtypes = [C.c_float]
+        L.oracle_render.restype = None
+        L.oracle_render.argtypes = [f32p, C.POINTER(C.c_uint16), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_uint8)]
         L.oracle_potential.restype = C.c_double
         L.oracle_potential.argtypes = [f32p, C.c_double]
         L.oracle_dpotential.restype = C.c_double
@@ -232,3 +235,18 @@ def eos_words(words, eos=0, eos_gamma=4.0 / 3.0, dof_ratio=1.0):
     w[23] = dof_ratio
     return w
 
+
+def render(view_words, texels, width: int, height: int) -> np.ndarray:
+    """volume.wgsl's fs_main on the CPU: view_words are the 44 floats of the
+    view uniform, texels the half-precision channels, shape (nz, ny, nx, 4);
+    returns RGBA8 pixels of shape (height, width, 4), row 0 at the top."""
+    v, vp = _f32(np.asarray(view_words, dtype=np.float32).reshape(44))
+    t = np.ascontiguousarray(texels)
+    if t.dtype != np.uint16:
+        t = np.ascontiguousarray(t.astype(np.float16)).view(np.uint16)
+    nz, ny, nx, c = t.shape
+    assert c == 4
+    out = np.zeros((height, width, 4), dtype=np.uint8)
+    lib().oracle_render(vp, t.ctypes.data_as(C.POINTER(C.c_uint16)), nx, ny, nz, width, height,
+                        out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out

@@ -15,6 +15,9 @@
 //   shaders/nucleate.wgsl   min_image, tanh profile, max-merge
 //   shaders/reduce.wgsl     cell_contribution; naive f32 two-pass summation
 //   shaders/vis.wgsl        visualisation channels (after the primitive pass)
+//   shaders/volume.wgsl     the ray march of fs_main over those channels, with
+//                           the trilinear clamp-to-edge filter of the sampler
+//                           (gpu/render.rs:184-196) written out
 //   src/gpu/sim.rs          RK3 (:135), stage_slots (:1046-1051), record_steps /
 //                           record_nucleation / record_step (:829-916)
 //
@@ -27,6 +30,7 @@
 #include <omp.h>
 #endif
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -566,6 +570,260 @@ void diag_naive_f32(Lattice<float> &L, float *out4, uint32_t *vmax_bits) {
 // ---------------------------------------------------------------------------
 //  ctypes surface
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+//  volume.wgsl -- fs_main per pixel, in f32, in the shader's expression order
+// ---------------------------------------------------------------------------
+namespace vol {
+
+struct V3 { float x, y, z; };
+inline V3 v3(float a, float b, float c) { return V3{a, b, c}; }
+inline V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 mulv(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline V3 muls(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 divv(V3 a, V3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+inline V3 divs(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline float dotv(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 normalize(V3 a) { return divs(a, std::sqrt(dotv(a, a))); }
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }  // WGSL mix
+inline V3 mixv(V3 a, V3 b, float t) { return v3(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)); }
+inline float clampf(float x, float lo, float hi) { return std::fmin(std::fmax(x, lo), hi); }
+inline float fract(float x) { return x - std::floor(x); }
+inline float smoothstep(float lo, float hi, float x) {
+    const float t = clampf((x - lo) / (hi - lo), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+const float VIRIDIS[11][3] = {{0.267f, 0.005f, 0.329f}, {0.283f, 0.141f, 0.458f}, {0.254f, 0.265f, 0.530f},
+                              {0.207f, 0.372f, 0.553f}, {0.164f, 0.471f, 0.558f}, {0.128f, 0.567f, 0.551f},
+                              {0.135f, 0.659f, 0.518f}, {0.267f, 0.749f, 0.441f}, {0.478f, 0.821f, 0.318f},
+                              {0.741f, 0.873f, 0.150f}, {0.993f, 0.906f, 0.144f}};
+const float INFERNO[11][3] = {{0.001f, 0.000f, 0.014f}, {0.078f, 0.044f, 0.214f}, {0.225f, 0.036f, 0.388f},
+                              {0.373f, 0.074f, 0.432f}, {0.522f, 0.128f, 0.420f}, {0.665f, 0.182f, 0.370f},
+                              {0.798f, 0.280f, 0.270f}, {0.902f, 0.413f, 0.144f}, {0.969f, 0.579f, 0.032f},
+                              {0.988f, 0.762f, 0.135f}, {0.988f, 0.998f, 0.645f}};
+const float SIGNED[11][3] = {{0.30f, 0.55f, 1.00f}, {0.22f, 0.42f, 0.85f}, {0.15f, 0.30f, 0.66f},
+                             {0.09f, 0.19f, 0.45f}, {0.04f, 0.09f, 0.24f}, {0.02f, 0.02f, 0.03f},
+                             {0.26f, 0.09f, 0.04f}, {0.48f, 0.17f, 0.06f}, {0.72f, 0.29f, 0.08f},
+                             {0.92f, 0.48f, 0.14f}, {1.00f, 0.72f, 0.32f}};
+
+// ramp (volume.wgsl:81-92)
+V3 ramp(float t, int which) {
+    const float x = clampf(t, 0.0f, 1.0f) * 10.0f;
+    const int i0 = (int)std::floor(x);
+    const int i1 = std::min(i0 + 1, 10);
+    const float f = x - std::floor(x);
+    const float (*c)[3] = which == 0 ? VIRIDIS : (which == 1 ? INFERNO : SIGNED);
+    return mixv(v3(c[i0][0], c[i0][1], c[i0][2]), v3(c[i1][0], c[i1][1], c[i1][2]), f);
+}
+
+// safe_inv (volume.wgsl:98-101)
+float safe_inv(float x) {
+    const float s = x < 0.0f ? -1.0f : 1.0f;
+    return s / std::fmax(std::fabs(x), 1e-9f);
+}
+
+float half_to_float(uint16_t h) {
+    const int sign = h >> 15, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    float v;
+    if (e == 0) v = std::ldexp((float)m, -24);
+    else if (e == 31) v = m ? NAN : INFINITY;
+    else v = std::ldexp((float)(m | 0x400), e - 25);
+    return sign ? -v : v;
+}
+
+struct S4 { float r, g, b, a; };
+
+struct Scene {
+    const float *V;  // the 44-float view uniform
+    const uint16_t *tex;
+    int n[3];
+    float vm(int block, int k) const { return V[16 + 4 * block + k]; }  // 0 cam, 1 half, 2 grid, 3..6 render0..3
+};
+
+S4 texel(const Scene &S, int x, int y, int z) {
+    const uint16_t *t = S.tex + 4 * (((size_t)z * S.n[1] + y) * S.n[0] + x);
+    return S4{half_to_float(t[0]), half_to_float(t[1]), half_to_float(t[2]), half_to_float(t[3])};
+}
+S4 lerp4(S4 a, S4 b, float t) { return S4{mixf(a.r, b.r, t), mixf(a.g, b.g, t), mixf(a.b, b.b, t), mixf(a.a, b.a, t)}; }
+
+// linear filtering with clamp-to-edge addressing: i0 = floor(u n - 1/2),
+// weight frac(u n - 1/2), both texels clamped into [0, n - 1]
+S4 sample_state(const Scene &S, V3 p) {
+    const V3 half = v3(S.vm(1, 0), S.vm(1, 1), S.vm(1, 2));
+    const V3 tc = add(muls(divv(p, half), 0.5f), v3(0.5f, 0.5f, 0.5f));  // to_texcoord
+    const float u[3] = {tc.x, tc.y, tc.z};
+    int lo[3], hi[3];
+    float w[3];
+    for (int a = 0; a < 3; ++a) {
+        const float c = u[a] * (float)S.n[a] - 0.5f;
+        const float f = std::floor(c);
+        w[a] = c - f;
+        lo[a] = std::min(std::max((int)f, 0), S.n[a] - 1);
+        hi[a] = std::min(std::max((int)f + 1, 0), S.n[a] - 1);
+    }
+    const S4 c00 = lerp4(texel(S, lo[0], lo[1], lo[2]), texel(S, hi[0], lo[1], lo[2]), w[0]);
+    const S4 c10 = lerp4(texel(S, lo[0], hi[1], lo[2]), texel(S, hi[0], hi[1], lo[2]), w[0]);
+    const S4 c01 = lerp4(texel(S, lo[0], lo[1], hi[2]), texel(S, hi[0], lo[1], hi[2]), w[0]);
+    const S4 c11 = lerp4(texel(S, lo[0], hi[1], hi[2]), texel(S, hi[0], hi[1], hi[2]), w[0]);
+    return lerp4(lerp4(c00, c10, w[1]), lerp4(c01, c11, w[1]), w[2]);
+}
+
+// grad_phi (volume.wgsl:131-141)
+V3 grad_phi(const Scene &S, V3 p) {
+    const float h = S.vm(1, 3);
+    const V3 dx = v3(h, 0.0f, 0.0f), dy = v3(0.0f, h, 0.0f), dz = v3(0.0f, 0.0f, h);
+    const V3 g = v3(sample_state(S, add(p, dx)).r - sample_state(S, sub(p, dx)).r,
+                    sample_state(S, add(p, dy)).r - sample_state(S, sub(p, dy)).r,
+                    sample_state(S, add(p, dz)).r - sample_state(S, sub(p, dz)).r);
+    return divs(g, 2.0f * h);
+}
+
+// hash12 (volume.wgsl:143-147)
+float hash12(float px, float py) {
+    V3 p3 = v3(fract(px * 0.1031f), fract(py * 0.1031f), fract(px * 0.1031f));
+    const float d = dotv(p3, add(v3(p3.y, p3.z, p3.x), v3(33.33f, 33.33f, 33.33f)));
+    p3 = add(p3, v3(d, d, d));
+    return fract((p3.x + p3.y) * p3.z);
+}
+
+inline uint8_t unorm8(float c) { return (uint8_t)(clampf(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
+
+void pixel(const Scene &S, int px, int py, int W, int H, uint8_t *out) {
+    const float *M = S.V;
+    const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+    const float nx = fx / (float)W * 2.0f - 1.0f, ny = 1.0f - fy / (float)H * 2.0f;
+    const V3 ro = v3(S.vm(0, 0), S.vm(0, 1), S.vm(0, 2));
+    // inv_view_proj * vec4(ndc, 1, 1), column-major
+    const float fw[4] = {M[0] * nx + M[4] * ny + M[8] * 1.0f + M[12] * 1.0f,
+                         M[1] * nx + M[5] * ny + M[9] * 1.0f + M[13] * 1.0f,
+                         M[2] * nx + M[6] * ny + M[10] * 1.0f + M[14] * 1.0f,
+                         M[3] * nx + M[7] * ny + M[11] * 1.0f + M[15] * 1.0f};
+    const V3 rd = normalize(sub(divs(v3(fw[0], fw[1], fw[2]), fw[3]), ro));
+    const float horizon = clampf(rd.y * 0.5f + 0.5f, 0.0f, 1.0f);
+    V3 color = muls(mixv(v3(0.015f, 0.017f, 0.028f), v3(0.045f, 0.055f, 0.085f), horizon), S.vm(6, 2));
+    auto emit_raw = [&](V3 c) {
+        out[0] = unorm8(c.x); out[1] = unorm8(c.y); out[2] = unorm8(c.z); out[3] = 255;
+    };
+    const V3 half = v3(S.vm(1, 0), S.vm(1, 1), S.vm(1, 2));
+    const V3 inv_rd = v3(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+    // intersect_box (volume.wgsl:104-113)
+    const V3 t0 = mulv(sub(v3(-half.x, -half.y, -half.z), ro), inv_rd);
+    const V3 t1 = mulv(sub(half, ro), inv_rd);
+    const V3 tmin = v3(std::fmin(t0.x, t1.x), std::fmin(t0.y, t1.y), std::fmin(t0.z, t1.z));
+    const V3 tmax = v3(std::fmax(t0.x, t1.x), std::fmax(t0.y, t1.y), std::fmax(t0.z, t1.z));
+    float t_near = std::fmax(std::fmax(std::fmax(tmin.x, tmin.y), tmin.z), 0.0f);
+    float t_far = std::fmin(std::fmin(tmax.x, tmax.y), tmax.z);
+    if (t_near >= t_far) return emit_raw(color);
+    const int clip_axis = (int)S.vm(4, 1);
+    if (clip_axis >= 0) {
+        const V3 n_axis = v3(clip_axis == 0 ? 1.0f : 0.0f, clip_axis == 1 ? 1.0f : 0.0f, clip_axis == 2 ? 1.0f : 0.0f);
+        const float plane = S.vm(4, 2) * S.vm(1, clip_axis);
+        const float denom = dotv(rd, n_axis);
+        const float dist = plane - dotv(ro, n_axis);
+        if (std::fabs(denom) < 1e-6f) {
+            if (dist < 0.0f) return emit_raw(color);
+        } else {
+            const float t_plane = dist / denom;
+            if (denom > 0.0f) t_far = std::fmin(t_far, t_plane);
+            else t_near = std::fmax(t_near, t_plane);
+            if (t_near >= t_far) return emit_raw(color);
+        }
+    }
+    if (S.vm(6, 1) > 0.5f) {
+        const float w = 0.004f * S.vm(1, 3) * S.vm(2, 0);
+        float edge = 0.0f;
+        for (int k = 0; k < 2; ++k) {
+            const V3 p = add(ro, muls(rd, k == 0 ? t_near : t_far));
+            const V3 d = sub(half, v3(std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)));
+            const float median = std::fmax(std::fmin(d.x, d.y), std::fmin(std::fmax(d.x, d.y), d.z));
+            edge = std::fmax(edge, 1.0f - smoothstep(0.0f, w, median));
+        }
+        color = add(color, muls(v3(0.20f, 0.26f, 0.34f), edge));
+    }
+    const float cell = S.vm(1, 3);
+    const float samples_per_cell = std::fmax(S.vm(3, 2), 0.25f);
+    float ds = cell / samples_per_cell;
+    const float span = t_far - t_near;
+    if (span / ds > 2048.0f) ds = span / 2048.0f;
+    const int n_steps = std::min((int)std::ceil(span / ds), 2048);
+    const float jitter = hash12(fx + S.vm(6, 0), fy + S.vm(6, 0));
+    const float iso = S.vm(3, 0), wall_opacity = S.vm(3, 1), exposure = S.vm(3, 3);
+    const float absorption = S.vm(5, 1);
+    const bool show_iso = S.vm(5, 2) > 0.5f, show_vol = S.vm(5, 3) > 0.5f;
+    const int mode = (int)S.vm(4, 0);
+    const float gain = S.vm(5, 0);
+    V3 accum = v3(0.0f, 0.0f, 0.0f);
+    float alpha = 0.0f;
+    float t = t_near + jitter * ds;
+    float prev_phi = sample_state(S, add(ro, muls(rd, t_near))).r;
+    float prev_t = t_near;
+    for (int i = 0; i < n_steps; ++i) {
+        if (t > t_far || alpha > 0.995f) break;
+        const V3 p = add(ro, muls(rd, t));
+        const S4 s = sample_state(S, p);
+        const float d_prev = prev_phi - iso, d_cur = s.r - iso;
+        if (show_iso && d_prev != 0.0f && d_prev * d_cur <= 0.0f) {
+            const float denom = s.r - prev_phi;
+            const float frac = std::fabs(denom) > 1e-6f ? (iso - prev_phi) / denom : 0.5f;
+            const float t_hit = mixf(prev_t, t, clampf(frac, 0.0f, 1.0f));
+            const V3 p_hit = add(ro, muls(rd, t_hit));
+            const V3 g = grad_phi(S, p_hit);
+            const V3 n = normalize(sub(v3(-g.x, -g.y, -g.z), muls(rd, 1e-6f)));
+            const S4 hs = sample_state(S, p_hit);
+            const float drive = clampf(hs.g * 4.0f, 0.0f, 1.0f);
+            const V3 base = mixv(v3(0.16f, 0.62f, 0.72f), v3(1.00f, 0.72f, 0.28f), drive);
+            const V3 l1 = normalize(v3(0.55f, 0.72f, 0.42f));
+            const V3 l2 = normalize(v3(-0.5f, -0.2f, -0.7f));
+            const V3 view = v3(-rd.x, -rd.y, -rd.z);
+            const float diff = std::fmax(dotv(n, l1), 0.0f) + 0.35f * std::fmax(dotv(n, l2), 0.0f);
+            const V3 h = normalize(add(add(l1, view), v3(0.0f, 1e-5f, 0.0f)));
+            const float spec = std::pow(std::fmax(dotv(n, h), 0.0f), 48.0f);
+            const float fres = std::pow(1.0f - std::fabs(dotv(n, view)), 3.0f);
+            const V3 shaded = add(add(muls(base, 0.22f + 0.85f * diff), muls(muls(v3(1.0f, 1.0f, 1.0f), spec), 0.35f)),
+                                  muls(muls(base, fres), 0.9f));
+            const float a = wall_opacity;
+            accum = add(accum, muls(muls(shaded, 1.0f - alpha), a));
+            alpha = alpha + (1.0f - alpha) * a;
+        }
+        if (show_vol) {
+            // map_field (volume.wgsl:156-186)
+            float mt, intensity;
+            int ramp_id = (int)S.vm(4, 3);
+            if (mode == 0) {
+                const float x = clampf(s.r, 0.0f, 1.0f);
+                mt = x; intensity = x * gain;
+            } else if (mode == 1) {
+                const float x = clampf(s.g * gain, 0.0f, 1.0f);
+                mt = x; intensity = x;
+            } else if (mode == 2) {
+                const float x = clampf(s.b * gain, -1.0f, 1.0f);
+                mt = 0.5f + 0.5f * x; intensity = std::fabs(x); ramp_id = 2;
+            } else {
+                const float x = clampf(s.a * gain, 0.0f, 1.0f);
+                mt = x; intensity = x;
+            }
+            if (intensity > 1e-4f) {
+                const float weight = intensity * intensity;
+                const float a = 1.0f - std::exp(-absorption * weight * ds);
+                const V3 emit = muls(ramp(mt, ramp_id), exposure);
+                accum = add(accum, muls(muls(emit, 1.0f - alpha), a));
+                alpha = alpha + (1.0f - alpha) * a;
+            }
+        }
+        prev_phi = s.r;
+        prev_t = t;
+        t = t + ds;
+    }
+    color = add(muls(color, 1.0f - alpha), accum);
+    color = divv(color, add(v3(1.0f, 1.0f, 1.0f), color));
+    color = v3(std::pow(std::fmax(color.x, 0.0f), 1.0f / 2.2f), std::pow(std::fmax(color.y, 0.0f), 1.0f / 2.2f),
+               std::pow(std::fmax(color.z, 0.0f), 1.0f / 2.2f));
+    emit_raw(color);
+}
+
+}  // namespace vol
+
 extern "C" {
 
 // Worker threads of the OpenMP loops that follow (n >= 1); returns the count
@@ -720,6 +978,16 @@ float oracle_limit_slope_f32(float dm, float dp) { return limit_slope(dm, dp); }
 
 double oracle_tanh_f64(double x) { return o_tanh(x); }
 float oracle_tanh_f32(float x) { return o_tanh(x); }
+
+// volume.wgsl fs_main for every pixel of a width x height frame (row 0 at
+// the top), from the 44-float view uniform and 4 half texels per cell
+void oracle_render(const float *view, const uint16_t *texels, int nx, int ny, int nz, int width, int height,
+                   uint8_t *rgba) {
+    const vol::Scene S{view, texels, {nx, ny, nz}};
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) vol::pixel(S, x, y, width, height, rgba + 4 * ((size_t)y * width + x));
+}
 
 double oracle_potential(const float *params, double phi) { return potential(make_prm<double>(params), phi); }
 double oracle_dpotential(const float *params, double phi) { return dpotential(make_prm<double>(params), phi); }

@@ -44,11 +44,27 @@ def test_usage_errors(bs, args):
     assert "error:" in r.stderr
 
 
-def test_interactive_and_screenshot_are_out_of_scope(bs):
+def test_interactive_window_is_out_of_scope(bs):
     r = cli(bs)
     assert r.returncode == 1 and "--headless" in r.stderr
-    r = cli(bs, "--headless", "10", "--screenshot", "x.png")
-    assert r.returncode == 1 and "renderer" in r.stderr
+
+
+@pytest.mark.parametrize("size,msg", [
+    ("1920", "`1920` is not a size; expected WIDTHxHEIGHT"),
+    ("0x600", "width must be positive"),
+    ("800x0", "height must be positive"),
+    ("axb", "width: invalid digit found in string"),
+])
+def test_screenshot_size_errors(bs, size, msg):
+    # parse_size (config.rs:181-192, the reference's size_parsing test)
+    r = cli(bs, "--headless", "1", "--screenshot", "x.png", "--screenshot-size", size)
+    assert r.returncode == 2
+    assert msg in r.stderr and "--screenshot-size" in r.stderr
+
+
+def test_field_must_be_a_field_mode(bs):
+    r = cli(bs, "--field", "pressure", "--headless", "1")
+    assert r.returncode == 2 and "--field" in r.stderr
 
 
 ROW = re.compile(r"^ {0,9}\d+ {1,10}\d+\.\d {1,9}\d+\.\d\d% {1,10}\d+\.\d {1,11}\d\.\d{4} {1,11}-?\d\.\d{3}e-?\d+ +"
