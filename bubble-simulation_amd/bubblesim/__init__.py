@@ -222,6 +222,7 @@ _SIGS = {
     "bs_batch_tile": (C.c_int, [_P, _u32p, _u32p]),
     "bs_batch_plan": (C.c_int, [_P, _u32p]),
     "bs_batch_graph_steps": (C.c_uint64, [_P]),
+    "bs_batch_stamps_skipped": (C.c_uint64, [_P]),
     "bs_batch_streams": (C.c_uint32, [_P]),
     "bs_server_create": (C.c_int, [C.POINTER(CServerConfig), C.POINTER(_P)]),
     "bs_server_submit": (C.c_int, [_P, C.POINTER(CJob), C.POINTER(C.c_uint64)]),
@@ -714,6 +715,10 @@ class Batch:
     def graph_steps(self) -> int:
         """Steps advanced so far through replay of a captured hipGraph."""
         return int(lib().bs_batch_graph_steps(self._h))
+
+    def stamps_skipped(self) -> int:
+        """Steps advanced without the nucleation stamp launch (no event due)."""
+        return int(lib().bs_batch_stamps_skipped(self._h))
 
     def streams(self) -> int:
         """Stream groups the boxes are spread over."""

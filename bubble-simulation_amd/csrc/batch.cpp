@@ -170,11 +170,14 @@ int bs_batch_advance(bs_batch *b, uint64_t n, int32_t use_graph) {
     if (!b) return set_error(BS_ERR_INVALID, "null batch");
     int rc = batch_sync_progress(b);
     if (rc) return rc;
-    b->progress_valid = false;
-    // every group's launches are enqueued before any is waited for
+    // every group's launches are enqueued before any is waited for; run()
+    // keeps the host mirrors in step, so the next call needs no read-back
     for (Group &g : b->groups) {
         rc = g.eng->run(n, use_graph != 0, kBatchGraphSteps);
-        if (rc) return rc;
+        if (rc) {
+            b->progress_valid = false;  // some launches may be queued
+            return rc;
+        }
     }
     return BS_OK;
 }
@@ -262,6 +265,13 @@ uint64_t bs_batch_graph_steps(const bs_batch *b) {
     if (!b || b->groups.empty()) return 0;
     uint64_t m = UINT64_MAX;
     for (const Group &g : b->groups) m = std::min(m, g.eng->graph_replayed);
+    return m;
+}
+
+uint64_t bs_batch_stamps_skipped(const bs_batch *b) {
+    if (!b || b->groups.empty()) return 0;
+    uint64_t m = UINT64_MAX;
+    for (const Group &g : b->groups) m = std::min(m, g.eng->stamps_skipped);
     return m;
 }
 

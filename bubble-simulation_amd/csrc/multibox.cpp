@@ -90,24 +90,28 @@ Engine::~Engine() {
 }
 
 void Engine::drop_graph() {
-    if (gexec || graph) {
-        // a launch of this graph may still be queued: destroy it only once
-        // the stream has passed this point
+    if (gexec[0] || graph[0] || gexec[1] || graph[1]) {
+        // a launch of these graphs may still be queued: destroy them only
+        // once the stream has passed this point
         hipEvent_t ev = nullptr;
         if (stream && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
             hipEventRecord(ev, stream) == hipSuccess) {
-            graveyard.push_back(Dead{graph, gexec, ev});
+            graveyard.push_back(Dead{{graph[0], graph[1]}, {gexec[0], gexec[1]}, ev});
         } else {
             if (ev) (void)hipEventDestroy(ev);
             (void)hipGetLastError();
             if (stream) (void)hipStreamSynchronize(stream);
-            if (gexec) (void)hipGraphExecDestroy(gexec);
-            if (graph) (void)hipGraphDestroy(graph);
+            for (int v = 0; v < 2; ++v) {
+                if (gexec[v]) (void)hipGraphExecDestroy(gexec[v]);
+                if (graph[v]) (void)hipGraphDestroy(graph[v]);
+            }
         }
     }
-    gexec = nullptr;
-    graph = nullptr;
-    graph_steps = 0;
+    for (int v = 0; v < 2; ++v) {
+        gexec[v] = nullptr;
+        graph[v] = nullptr;
+        graph_steps[v] = 0;
+    }
     reap_graphs(false);
 }
 
@@ -119,8 +123,10 @@ void Engine::reap_graphs(bool wait) {
             ++i;
             continue;
         }
-        if (d.x) (void)hipGraphExecDestroy(d.x);
-        if (d.g) (void)hipGraphDestroy(d.g);
+        for (int v = 0; v < 2; ++v) {
+            if (d.x[v]) (void)hipGraphExecDestroy(d.x[v]);
+            if (d.g[v]) (void)hipGraphDestroy(d.g[v]);
+        }
         (void)hipEventDestroy(d.after);
         graveyard.erase(graveyard.begin() + (long)i);
     }
@@ -285,66 +291,85 @@ template <class R> static hipError_t launch_stage_all(const Engine &e, int s) {
     return hipSuccess;
 }
 
-template <class R> static int step_eager_t(Engine &e) {
+template <class R> static int step_eager_t(Engine &e, bool stamp) {
     const int n = (int)e.entries.size();
     Box<R> *db = (Box<R> *)e.d_boxes;
-    BS_HIP_TRY(launch_stamp_multi<R>(db, e.d_events, n, e.stream));
+    if (stamp) BS_HIP_TRY(launch_stamp_multi<R>(db, e.d_events, n, e.stream));
     for (int s = 0; s < 3; ++s) BS_HIP_TRY(launch_stage_all<R>(e, s));
     BS_HIP_TRY(launch_advance_multi<R>(db, e.d_events, n, e.stream));
     return BS_OK;
 }
 
-int Engine::step_eager() {
+int Engine::step_eager(bool stamp) {
     if (entries.empty()) return BS_OK;
-    return precision == BS_PRECISION_F64 ? step_eager_t<double>(*this) : step_eager_t<float>(*this);
+    return precision == BS_PRECISION_F64 ? step_eager_t<double>(*this, stamp) : step_eager_t<float>(*this, stamp);
+}
+
+// k_stamp_multi stamps a box at a step only when its next event is due
+// (step[next] <= steps_done, with steps to go); without such a box in the
+// next k steps it would write nothing, and leaving it out changes nothing
+bool Engine::event_due(uint64_t k) const {
+    if (k == 0) return false;
+    for (const Entry *e : entries) {
+        if (e->steps_done >= e->steps_target || e->next_event >= (int)e->h_steps.size()) continue;
+        if (e->h_steps[e->next_event] <= e->steps_done + (k - 1)) return true;
+    }
+    return false;
 }
 
 int Engine::run(uint64_t n, bool use_graph, int graph_chunk) {
     if (entries.empty() || n == 0) return BS_OK;
-    if (!use_graphs || !use_graph || graph_chunk <= 1 || n < (uint64_t)graph_chunk) {
-        for (uint64_t i = 0; i < n; ++i) {
-            int rc = step_eager();
+    auto advance_mirrors = [&](uint64_t k) {
+        for (Entry *e : entries) e->host_advance(k);
+    };
+    auto eager = [&](uint64_t k) -> int {
+        for (uint64_t i = 0; i < k; ++i) {
+            const bool stamp = event_due(1);
+            int rc = step_eager(stamp);
             if (rc) return rc;
+            if (!stamp) stamps_skipped += 1;
+            advance_mirrors(1);
         }
         return BS_OK;
-    }
-    if (!gexec || graph_steps != graph_chunk) {
-        drop_graph();
-        // relaxed: the capture holds kernel launches only, and other threads
-        // (a server's retire thread) keep using their own streams meanwhile
-        BS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-        int rc = BS_OK;
-        for (int i = 0; i < graph_chunk && !rc; ++i) rc = step_eager();
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return set_error(BS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        }
-        e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            gexec = nullptr;
-            return set_error(BS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        }
-        graph = g;
-        graph_steps = graph_chunk;
-    }
+    };
+    if (!use_graphs || !use_graph || graph_chunk <= 1 || n < (uint64_t)graph_chunk) return eager(n);
     uint64_t done = 0;
-    while (n - done >= (uint64_t)graph_steps) {
-        BS_HIP_TRY(hipGraphLaunch(gexec, stream));
-        done += (uint64_t)graph_steps;
-        graph_replayed += (uint64_t)graph_steps;
+    while (n - done >= (uint64_t)graph_chunk) {
+        const int v = event_due((uint64_t)graph_chunk) ? 1 : 0;
+        if (!gexec[v] || graph_steps[v] != graph_chunk) {
+            if (gexec[v] || graph[v]) drop_graph();  // the chunk length changed
+            // relaxed: the capture holds kernel launches only, and other
+            // threads (a server's retire thread) keep using their own
+            // streams meanwhile
+            BS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            int rc = BS_OK;
+            for (int i = 0; i < graph_chunk && !rc; ++i) rc = step_eager(v == 1);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return set_error(BS_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            }
+            e = hipGraphInstantiate(&gexec[v], g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                gexec[v] = nullptr;
+                return set_error(BS_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            graph[v] = g;
+            graph_steps[v] = graph_chunk;
+        }
+        BS_HIP_TRY(hipGraphLaunch(gexec[v], stream));
+        done += (uint64_t)graph_chunk;
+        graph_replayed += (uint64_t)graph_chunk;
+        if (!v) stamps_skipped += (uint64_t)graph_chunk;
+        advance_mirrors((uint64_t)graph_chunk);
     }
-    for (; done < n; ++done) {
-        int rc = step_eager();
-        if (rc) return rc;
-    }
-    return BS_OK;
+    return eager(n - done);
 }
 
 int Engine::time_stages(float ms[2]) {

@@ -72,15 +72,18 @@ struct Engine {
     size_t stage_cap[2] = {0, 0};
     hipEvent_t stage_done[2] = {nullptr, nullptr};
     int stage_next = 0;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_steps = 0;
+    // captured chunks of steps, [0] without and [1] with the nucleation
+    // stamp ahead of each step
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
     uint64_t graph_replayed = 0;  // steps run through graph replay
+    uint64_t stamps_skipped = 0;  // steps run without the stamp launch
     bool use_graphs = true;
     // dropped graphs, destroyed once the stream has passed their last launch
     struct Dead {
-        hipGraph_t g;
-        hipGraphExec_t x;
+        hipGraph_t g[2];
+        hipGraphExec_t x[2];
         hipEvent_t after;
     };
     std::vector<Dead> graveyard;
@@ -92,9 +95,15 @@ struct Engine {
     int upload();
     // read device progress (steps_done, next, cur) back into the entries
     int download_progress();
-    // one step of every entry (eager)
-    int step_eager();
-    // n steps, through a captured graph of `graph_chunk` steps when use_graph
+    // whether any entry passes an event in its next k steps (from the host
+    // mirrors, which follow the device exactly)
+    bool event_due(uint64_t k) const;
+    // one step of every entry (eager), with or without the stamp launch
+    int step_eager(bool stamp);
+    // n steps, through a captured graph of `graph_chunk` steps when
+    // use_graph; the stamp launch is left out of every step (or chunk) in
+    // which no entry has an event due.  Keeps the host mirrors of every
+    // entry in step with the launches enqueued.
     int run(uint64_t n, bool use_graph, int graph_chunk);
     // mean time of one launch of each stage class over the current set, in
     // ms (stage 1 alone; stages 2 and 3 as a pair, per launch)

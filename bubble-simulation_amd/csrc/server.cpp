@@ -265,9 +265,8 @@ static int tick(bs_server *s, uint32_t tick_id, std::vector<char> &bad) {
     }
     const bool reuse = !s->set_changed;
     s->set_changed = false;
-    rc = s->eng.run(s->cfg.chunk_steps, reuse, (int)s->cfg.chunk_steps);
+    rc = s->eng.run(s->cfg.chunk_steps, reuse, (int)s->cfg.chunk_steps);  // advances the host mirrors too
     if (rc) return rc;
-    for (auto &j : s->active) j->entry->host_advance(s->cfg.chunk_steps);
     rc = s->eng.check_finite(tick_id);
     if (rc) return rc;
     const size_t n = s->active.size();

@@ -513,6 +513,12 @@ int bs_batch_plan(const bs_batch *b, uint32_t out[6]);
 /* Steps advanced so far through replay of a captured hipGraph (25 steps per
  * graph; bs_batch_advance steps a remainder below that eagerly). */
 uint64_t bs_batch_graph_steps(const bs_batch *b);
+/* Steps advanced so far without the nucleation stamp launch: a step (or a
+ * 25-step graph) in which no box of a stream group has an event due leaves
+ * the stamp kernel out, as it would write nothing (least over the groups).
+ * Simulation::record_nucleation (sim.rs:843-860) likewise records no
+ * nucleation pass when its batch of due events is empty. */
+uint64_t bs_batch_stamps_skipped(const bs_batch *b);
 /* Streams the batch's boxes are spread over: groups of about equal cell
  * count, each stepped by its own launches (and graph) on its own stream, so
  * that the groups' launches overlap on the device. */

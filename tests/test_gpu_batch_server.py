@@ -70,6 +70,49 @@ def test_batch_graph_equals_eager_fast(bs, gpu_available):
     c.close()
 
 
+def test_batch_skips_the_stamp_only_when_nothing_is_due(bs, gpu_available):
+    # events spread over steps 0..90: of the four 25-step graphs some carry the
+    # stamp and some do not (each variant is captured once and replayed), the
+    # eager remainder stamps step by step; every box still equals its solo run
+    rng = np.random.default_rng(21)
+    specs = []
+    for k in range(4):
+        grid = (int(rng.choice([16, 24, 32])), 24, int(rng.choice([16, 20])))
+        specs.append(bs.make_spec(grid=grid, model=bs.Model(eta=0.1, seed_factor=1.2), nucleation=bs.NUC_UNIFORM_STEPS,
+                                  bubbles=2, seed=k + 7, uniform_max_step=90, time_seed=40 + k, math=bs.MATH_EXACT))
+    b = bs.Batch(specs)
+    b.advance(110, use_graph=True)
+    assert b.graph_steps() == 100
+    print(f"stamps skipped: {b.stamps_skipped()} of 110 steps", flush=True)
+    for i, s in enumerate(specs):
+        sim = bs.Simulation(s)
+        sim.advance(110)
+        f, u = sim.download()
+        bf, bu = b.download(i)
+        assert b.steps(i) == 110
+        assert np.array_equal(f, bf) and np.array_equal(u, bu), f"box {i}"
+        sim.close()
+    b.close()
+    # simultaneous nucleation: every event is due at step 0, so the first
+    # graph stamps and the second graph and the 10 eager steps do not
+    specs = [bs.make_spec(grid=(24, 24, 16), nucleation=bs.NUC_SIMULTANEOUS, bubbles=3, seed=k + 1, math=bs.MATH_EXACT)
+             for k in range(3)]
+    b = bs.Batch(specs)
+    b.advance(60, use_graph=True)
+    assert b.graph_steps() == 50
+    assert b.stamps_skipped() == 35
+    b.advance(5, use_graph=False)
+    assert b.stamps_skipped() == 40
+    for i, s in enumerate(specs):
+        sim = bs.Simulation(s)
+        sim.advance(65)
+        ref = sim.download()
+        got = b.download(i)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), f"box {i}"
+        sim.close()
+    b.close()
+
+
 def _wait_all(srv, ids, timeout=300):
     return {j: srv.wait(j, timeout) for j in ids}
 
