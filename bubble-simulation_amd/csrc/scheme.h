@@ -348,13 +348,16 @@ __host__ __device__ inline V4<R> sanitise_v2(const V4<R> &q, const Params<R> &P,
 // sanitise_v2 of the two states of one face.  FastMath tests both against
 // the cap in one branch, which no wave takes unless one of its states is
 // clamped; inside it each state is scaled by its own factor (1 when it is
-// not clamped), so every value is the one sanitise_v2 gives.
+// not clamped), so every velocity is the one sanitise_v2 gives.  FastMath
+// leaves out the pressure floor: the states are q +- h with the MC half
+// slope |h| <= min(|q - q_-|, |q_+ - q|) of equal sign, so each lies between
+// its cell and the neighbour across the face, both recovered with p >=
+// p_floor; rounding can leave a few ulps below it, still positive, which is
+// all the flux needs.
 template <class M, class R>
 __host__ __device__ inline void sanitise_pair_v2(V4<R> &a, R &v2a, V4<R> &b, R &v2b, const Params<R> &P) {
     if constexpr (M::fast) {
         const R vmax2 = P.z_max * P.z_max;
-        a.w = M::max(a.w, P.p_floor);
-        b.w = M::max(b.w, P.p_floor);
         v2a = dot3(a.x, a.y, a.z, a.x, a.y, a.z);
         v2b = dot3(b.x, b.y, b.z, b.x, b.y, b.z);
         const bool ca = v2a > vmax2, cb = v2b > vmax2;
