@@ -3,7 +3,9 @@
 // Keeps the reference's flags (config.rs:13-115) and its headless report
 // (headless.rs:23-116) byte for byte where the numbers agree, on top of the C
 // ABI.  Extensions: --precision, --math, --kernel, --bubble-at-centre,
-// --nucleation uniform-steps with --uniform-max-step/--time-seed, --dump.
+// --nucleation uniform-steps with --uniform-max-step/--time-seed, --dump,
+// --check-f64 (the same run in f64 ExactMath beside it: field error and
+// energy drift of the result against it).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -66,6 +68,8 @@ void usage(FILE *f) {
                  "      --eos-gamma <G>                gamma-law adiabatic index, 1 < G < 2 [default: 4/3]\n"
                  "      --dof-ratio <R>                a_broken / a_symmetric of the temperature [default: 1]\n"
                  "      --dump <PATH>                  Write the final state (field then fluid, raw) to PATH\n"
+                 "      --check-f64                    Step an f64 ExactMath copy of the run beside it; report the\n"
+                 "                                     relative L2/Linf error of the result and both energy drifts\n"
                  "  -h, --help                         Print help\n"
                  "  -V, --version                      Print version\n");
 }
@@ -117,6 +121,86 @@ bool parse_size(const std::string &s, uint32_t &w, uint32_t &h, std::string &err
     return true;
 }
 
+// Relative errors of a state block (k components per cell) against the f64
+// run: sqrt(sum d^2 / sum r^2) and max|d| / max|r| over every component,
+// accumulated in double.
+struct BlockError {
+    double l2 = 0.0, linf = 0.0;
+};
+BlockError block_error(const float *a, const double *r, size_t count) {
+    double se = 0.0, sr = 0.0, me = 0.0, mr = 0.0;
+    for (size_t i = 0; i < count; ++i) {
+        const double d = (double)a[i] - r[i];
+        se += d * d;
+        sr += r[i] * r[i];
+        me = std::fmax(me, std::fabs(d));
+        mr = std::fmax(mr, std::fabs(r[i]));
+    }
+    BlockError e;
+    e.l2 = sr > 0.0 ? std::sqrt(se / sr) : 0.0;
+    e.linf = mr > 0.0 ? me / mr : 0.0;
+    return e;
+}
+
+// --check-f64: the same run in f64 ExactMath, after the timed one, with the
+// same report steps; prints the final state's error and the energy drift of
+// both runs from the first report after the last nucleation (headless.rs:54-78)
+int check_against_f64(const bs_spec &spec, bs_sim *sim, uint64_t steps, uint64_t every,
+                      const std::vector<double> &energies, bool have_base, size_t base_idx, uint64_t base_step) {
+    bs_spec s64 = spec;
+    s64.precision = BS_PRECISION_F64;
+    s64.math = BS_MATH_EXACT;
+    bs_sim *ref = nullptr;
+    if (bs_sim_create(&s64, &ref) != BS_OK) {
+        std::fprintf(stderr, "Error: the f64 copy of the run: %s\n", bs_last_error());
+        return 1;
+    }
+    std::vector<double> e64;
+    uint64_t done = 0;
+    int rc = 0;
+    while (done < steps && !rc) {
+        const uint64_t chunk = every < steps - done ? every : steps - done;
+        bs_diagnostics diag;
+        if (bs_sim_advance(ref, chunk) != BS_OK || bs_sim_diagnostics(ref, &diag) != BS_OK) {
+            std::fprintf(stderr, "Error: the f64 copy of the run: %s\n", bs_last_error());
+            rc = 1;
+        } else {
+            done += chunk;
+            e64.push_back(diag.mean_energy);
+        }
+    }
+    if (!rc) {
+        const uint64_t n = bs_sim_cell_count(sim);
+        std::vector<float> a(n * 6);
+        std::vector<double> r(n * 6);
+        if (bs_sim_download(sim, a.data(), a.data() + n * 2) != BS_OK ||
+            bs_sim_download(ref, r.data(), r.data() + n * 2) != BS_OK) {
+            std::fprintf(stderr, "Error: %s\n", bs_last_error());
+            rc = 1;
+        } else {
+            const BlockError ef = block_error(a.data(), r.data(), n * 2);
+            const BlockError eu = block_error(a.data() + n * 2, r.data() + n * 2, n * 4);
+            std::printf("\nf64 check after %llu steps (f64 ExactMath run of the same spec):\n", (unsigned long long)steps);
+            std::printf("  field (phi, pi): rel L2 %.3e, rel Linf %.3e\n", ef.l2, ef.linf);
+            std::printf("  fluid (E, Z):    rel L2 %.3e, rel Linf %.3e\n", eu.l2, eu.linf);
+            // the base report is the same in both runs: they share the schedule
+            const size_t b = base_idx;
+            if (have_base && b < energies.size() && energies.size() == e64.size()) {
+                double worst = 0.0;
+                for (size_t i = b; i < energies.size(); ++i)
+                    worst = std::fmax(worst, std::fabs((energies[i] / energies[b] - 1.0) - (e64[i] / e64[b] - 1.0)));
+                std::printf("  energy drift from step %llu: f32 %+.3e, f64 %+.3e (largest difference over the reports %.3e)\n",
+                            (unsigned long long)base_step, energies.back() / energies[b] - 1.0,
+                            e64.back() / e64[b] - 1.0, worst);
+            } else {
+                std::printf("  energy drift: no report after the last nucleation\n");
+            }
+        }
+    }
+    bs_sim_destroy(ref);
+    return rc;
+}
+
 int fail_usage(const std::string &msg) {
     std::fprintf(stderr, "error: %s\n\nFor more information, try '--help'.\n", msg.c_str());
     return 2;
@@ -138,6 +222,7 @@ int main(int argc, char **argv) {
     std::string screenshot_path;
     uint32_t shot_w = 1280, shot_h = 960;  // config.rs:112-114
     int32_t field_mode = 3;                // FieldMode::Kinetic (render.rs:26-28)
+    bool check_f64 = false;
 
     std::vector<std::string> args(argv + 1, argv + argc);
     for (size_t i = 0; i < args.size(); ++i) {
@@ -169,6 +254,8 @@ int main(int argc, char **argv) {
         } else if (a == "--bubble-at-centre") {
             spec.bubble_at_centre = 1;
             spec.bubbles = 1;
+        } else if (a == "--check-f64") {
+            check_f64 = true;
         } else if (!need(v)) {
             if (a.rfind("-", 0) == 0) return fail_usage("a value is required for '" + a + "' but none was supplied");
             return fail_usage("unexpected argument '" + a + "' found");
@@ -242,6 +329,8 @@ int main(int argc, char **argv) {
         }
     }
     if (spec.precision == BS_PRECISION_F64) spec.math = BS_MATH_EXACT;
+    if (check_f64 && spec.precision == BS_PRECISION_F64)
+        return fail_usage("'--check-f64' compares an f32 run with its f64 copy; the run is already f64");
     spec.model.seed_kind = have_seed_radius ? BS_SEED_CELLS : BS_SEED_CRITICAL;
     spec.model.seed_value = have_seed_radius ? seed_radius : seed_factor;
 
@@ -295,6 +384,9 @@ int main(int argc, char **argv) {
 
     bool have_base = false;
     double base = 0.0;
+    uint64_t base_step = 0;
+    size_t base_idx = 0;
+    std::vector<double> energies;  // mean energy at each report (for --check-f64)
     uint64_t done = 0;
     const auto start = std::chrono::steady_clock::now();
     int rc = 0;
@@ -322,7 +414,10 @@ int main(int argc, char **argv) {
         if (!have_base && bs_sim_remaining_bubbles(sim) == 0) {
             have_base = true;
             base = diag.mean_energy;
+            base_step = done;
+            base_idx = energies.size();
         }
+        energies.push_back(diag.mean_energy);
         char ratio[64];
         if (have_base) std::snprintf(ratio, sizeof ratio, "%.6f", diag.mean_energy / base);
         else std::snprintf(ratio, sizeof ratio, "-");
@@ -340,6 +435,7 @@ int main(int argc, char **argv) {
         std::printf("\n%llu steps in %.2f s  (%.1f steps/s, %.2f G cell-updates/s)\n", (unsigned long long)steps,
                     elapsed, (double)steps / elapsed, cell_steps / elapsed / 1e9);
     }
+    if (!rc && check_f64 && steps > 0) rc = check_against_f64(spec, sim, steps, every, energies, have_base, base_idx, base_step);
     if (!rc && !dump_path.empty()) {
         const size_t real = spec.precision == BS_PRECISION_F64 ? 8 : 4;
         const uint64_t n = bs_sim_cell_count(sim);

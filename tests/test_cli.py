@@ -67,6 +67,12 @@ def test_field_must_be_a_field_mode(bs):
     assert r.returncode == 2 and "--field" in r.stderr
 
 
+def test_check_f64_needs_an_f32_run(bs):
+    assert "--check-f64" in cli(bs, "--help").stdout
+    r = cli(bs, "--precision", "f64", "--check-f64", "--headless", "1")
+    assert r.returncode == 2 and "already f64" in r.stderr
+
+
 ROW = re.compile(r"^ {0,9}\d+ {1,10}\d+\.\d {1,9}\d+\.\d\d% {1,10}\d+\.\d {1,11}\d\.\d{4} {1,11}-?\d\.\d{3}e-?\d+ +"
                  r"(-|\d\.\d{6})$")
 
@@ -98,3 +104,29 @@ def test_default_report_every(bs, gpu_available):
     assert r.returncode == 0
     rows = [l for l in r.stdout.splitlines() if ROW.match(l)]
     assert len(rows) == 20  # max(steps / 20, 1) = 2 -> 20 reports (main.rs:32)
+
+
+@pytest.mark.gpu
+def test_check_f64_reports_error_and_drift(bs, gpu_available):
+    # the north star's "field error and energy drift against the f64 scheme
+    # for every result", for a headless run: FastMath f32 against its f64
+    # ExactMath copy after 120 steps (the last bubble is in at step 0)
+    r = cli(bs, "--grid", "48", "--bubbles", "2", "--nucleation", "simultaneous", "--headless", "120",
+            "--report-every", "20", "--check-f64")
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = r.stdout
+    assert "f64 check after 120 steps" in out
+    num = r"(\d\.\d{3}e[-+]\d\d)"
+    m_field = re.search(r"field \(phi, pi\): rel L2 " + num + ", rel Linf " + num, out)
+    m_fluid = re.search(r"fluid \(E, Z\):    rel L2 " + num + ", rel Linf " + num, out)
+    m_drift = re.search(r"energy drift from step 20: f32 ([-+]\d\.\d{3}e[-+]\d\d), f64 ([-+]\d\.\d{3}e[-+]\d\d) "
+                        r"\(largest difference over the reports (\d\.\d{3}e[-+]\d\d)\)", out)
+    assert m_field and m_fluid and m_drift, out
+    # block relative L2 of FastMath f32 against the f64 scheme: the field
+    # within the 1e-5 bar; the fluid block (E-dominated) within 1e-4 here
+    # (the bar with its norm is held at the benchmark configurations)
+    assert float(m_field.group(1)) < 1e-5 and float(m_fluid.group(1)) < 1e-4
+    # both drifts are small, and the f32 one follows the f64 one
+    d32, d64, worst = (float(m_drift.group(k)) for k in (1, 2, 3))
+    assert abs(d64) < 1e-3 and worst < 1e-5
