@@ -490,7 +490,10 @@ typedef struct bs_batch bs_batch;
 int bs_batch_create(const bs_spec *specs, uint32_t n, bs_batch **out);
 void bs_batch_destroy(bs_batch *b);
 /* Advance every box by n steps (advance semantics per box).  use_graph != 0
- * captures steps into a hipGraph and replays it. */
+ * captures steps into a hipGraph and replays it.  The steps are enqueued on
+ * the batch's streams and the call returns without waiting for them
+ * (bs_batch_synchronize waits; the other calls on a box order themselves
+ * after them). */
 int bs_batch_advance(bs_batch *b, uint64_t n, int32_t use_graph);
 int bs_batch_synchronize(bs_batch *b);
 uint32_t bs_batch_size(const bs_batch *b);
@@ -501,6 +504,8 @@ int bs_batch_download(bs_batch *b, uint32_t box, void *field, void *fluid);
  * so a box's state reaches no other box: this is how a host forces a failing
  * (e.g. non-finite) box to check that its neighbours are untouched. */
 int bs_batch_upload(bs_batch *b, uint32_t box, const void *field, const void *fluid);
+/* Steps of one box, counting those enqueued by bs_batch_advance (at most the
+ * box's step target), whether or not the device has finished them. */
 int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out);
 /* Tile shape chosen by the autotuner for RK stages 2 and 3 of this batch
  * (x, y). */
