@@ -84,7 +84,10 @@ int bs_device_name(char *buf, size_t len) {
         return bs::set_error(BS_ERR_DEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
     if (buf && len) {
-        std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+        // some drivers leave the marketing name empty; the architecture and
+        // CU count identify the part either way
+        const char *name = p.name[0] ? p.name : "AMD Instinct GPU";
+        std::snprintf(buf, len, "%s (%s, %d CUs)", name, p.gcnArchName, p.multiProcessorCount);
     }
     return BS_OK;
 }
