@@ -191,7 +191,12 @@ int download_state(BoxState &bx, hipStream_t stream, void *field, void *fluid) {
 
 int state_checksum(BoxState &bx, hipStream_t stream, uint64_t *out) {
     const size_t r = real_size(bx.precision);
-    std::vector<unsigned char> buf(bx.n_cells * 6 * r);
+    std::vector<unsigned char> buf;
+    try {
+        buf.resize(bx.n_cells * 6 * r);
+    } catch (const std::bad_alloc &) {
+        return set_error(BS_ERR_OOM, "no host memory for the state checksum");
+    }
     int rc = download_state(bx, stream, buf.data(), buf.data() + bx.n_cells * 2 * r);
     if (rc) return rc;
     uint64_t h = 1469598103934665603ULL;

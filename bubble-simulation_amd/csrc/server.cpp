@@ -131,7 +131,14 @@ static void retire_one(bs_server *s, Job &j) {
         }
         const size_t r = real_size(e->spec.precision);
         std::vector<unsigned char> buf;
-        if (j.status == BS_OK && (j.job.keep_state || j.job.checksum)) buf.resize(e->bx.n_cells * 6 * r);
+        if (j.status == BS_OK && (j.job.keep_state || j.job.checksum)) {
+            try {
+                buf.resize(e->bx.n_cells * 6 * r);
+            } catch (const std::bad_alloc &) {
+                j.status = BS_ERR_OOM;  // no host memory for the state: the job fails, the server goes on
+                d.res.status = j.status;
+            }
+        }
         if (!buf.empty() &&
             download_state(e->bx, s->rstream, buf.data(), buf.data() + e->bx.n_cells * 2 * r) == BS_OK) {
             uint64_t h = 1469598103934665603ULL;

@@ -426,12 +426,20 @@ int bs_sim_render(bs_sim *s, const bs_render_view *view, uint32_t width, uint32_
 int bs_sim_screenshot(bs_sim *s, const bs_render_settings *settings, uint32_t width, uint32_t height,
                       const char *path) {
     if (!s || !settings || !path) return set_error(BS_ERR_INVALID, "null argument");
+    // the frame limit of bs_sim_render, before the host buffer is sized
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return set_error(BS_ERR_INVALID, "the frame must be 1x1 to 16384x16384 pixels");
     bs_render_view view;
     int rc = bs_render_view_make(s->bx.grid, settings, width, height, 1, &view);
     if (rc) return rc;
     rc = bs_sim_record_visualisation(s);
     if (rc) return rc;
-    std::vector<uint8_t> px((size_t)width * height * 4);
+    std::vector<uint8_t> px;
+    try {
+        px.resize((size_t)width * height * 4);
+    } catch (const std::bad_alloc &) {
+        return set_error(BS_ERR_OOM, "no host memory for the frame");
+    }
     rc = bs_sim_render(s, &view, width, height, px.data());
     if (rc) return rc;
     return bs_png_write(path, px.data(), width, height);

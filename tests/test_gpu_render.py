@@ -127,3 +127,16 @@ def test_render_timing_at_the_default_size(bs, gpu_available):
         _log(f"{n}^3: {dt * 1e3:.2f} ms per 1280x960 frame (including the 4.9 MB copy to the host)")
         assert img.shape == (960, 1280, 4)
         sim.close()
+
+
+def test_screenshot_size_limit(bs, gpu_available, tmp_path):
+    # the frame limit is checked before any buffer is sized
+    sim = _state(bs, (32, 32, 32), 5)
+    st = bs.render_settings(bs.FIELD_PHI)
+    with pytest.raises(bs.BubbleSimError) as e:
+        sim.screenshot(str(tmp_path / "big.png"), st, (100000, 100000))
+    assert e.value.code == bs.BS_ERR_INVALID and "16384" in str(e.value)
+    sim.close()
+    r = subprocess.run([bs.CLI_PATH, "--grid", "32", "--headless", "2", "--screenshot", str(tmp_path / "c.png"),
+                        "--screenshot-size", "100000x100000"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "16384" in r.stderr
