@@ -108,9 +108,9 @@ def test_default_report_every(bs, gpu_available):
 
 @pytest.mark.gpu
 def test_check_f64_reports_error_and_drift(bs, gpu_available):
-    # the north star's "field error and energy drift against the f64 scheme
-    # for every result", for a headless run: FastMath f32 against its f64
-    # ExactMath copy after 120 steps (the last bubble is in at step 0)
+    # a headless run's field error and energy drift against the f64 scheme:
+    # FastMath f32 against its f64 ExactMath copy after 120 steps (the last
+    # bubble is in at step 0)
     r = cli(bs, "--grid", "48", "--bubbles", "2", "--nucleation", "simultaneous", "--headless", "120",
             "--report-every", "20", "--check-f64")
     print(r.stdout, r.stderr)
