@@ -143,7 +143,7 @@ TILES = ["32x16", "64x8", "32x8", "16x16", "64x4", "16x8", "32x4", "16x15"]
 
 
 @pytest.mark.parametrize("tile", TILES)
-@pytest.mark.parametrize("zc", ["16", "7"])
+@pytest.mark.parametrize("zc", ["16", "7", "4", "2"])
 def test_every_tile_shape_matches_oracle(bs, orc, gpu_available, monkeypatch, tile, zc):
     # the fused kernel's tiling (x/y tile, z chunk) changes which thread and
     # which launch evaluates a face, never its operands: bitwise under ExactMath
