@@ -455,7 +455,13 @@ int bs_server_submit(bs_server *s, const bs_job *job, uint64_t *job_id_out) {
     if (job->init_field) {
         const size_t cells = (size_t)sp.grid[0] * sp.grid[1] * sp.grid[2];
         const size_t r = real_size(sp.precision);
-        j->init.resize(cells * 6 * r);
+        try {
+            j->init.resize(cells * 6 * r);
+        } catch (const std::bad_alloc &) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stats.rejected += 1;
+            return set_error(BS_ERR_OOM, "no host memory for the job's initial state");
+        }
         std::memcpy(j->init.data(), job->init_field, cells * 2 * r);
         std::memcpy(j->init.data() + cells * 2 * r, job->init_fluid, cells * 4 * r);
     }
