@@ -113,21 +113,27 @@ def test_runaway_wall_128(bs, gpu_available):
 
 
 def test_courant_study_96(bs, gpu_available):
-    # docs/NUMERICS.md:213-222: "Both are stable; 0.5 is 13x less accurate."
-    # The published drifts are not reproduced (settings unknown, see the
-    # fixture's note); the qualitative claims are: both runs stay finite, the
-    # drift is negative, and cfl 0.5 is several times less accurate.
+    # docs/NUMERICS.md:213-222: 96^3, one bubble, eta = 0.02, to t = 450;
+    # drifts -8.2e-4 (cfl 0.2, 2250 steps) and -1.06e-2 (cfl 0.5, 900 steps),
+    # "0.5 is 13x less accurate".  The published drifts take E0 at the first
+    # of five reports (t = 90, just after the box has broken, where the drift
+    # falls steeply with the E0 time; the fixture's notes give the curves):
+    # reproduced to 13 % and 6 %, the ratio to 12x.
+    ref = PUB["courant_96"]
     drift = {}
-    for row in PUB["courant_96"]["rows"]:
+    for row in ref["rows"]:
         spec = bs.make_spec(grid=96, model=bs.Model(eta=0.02, cfl=row["cfl"]), nucleation=bs.NUC_SIMULTANEOUS,
                             bubbles=1, seed=1)
-        rows = headless_table(bs, spec, row["steps"], max(row["steps"] // 20, 1))
+        rows = headless_table(bs, spec, row["steps"], row["steps"] // ref["reports"])
         drift[row["cfl"]] = rows[-1]["e_ratio"] - 1
-        print(f"cfl {row['cfl']}: t = {rows[-1]['time']:.1f}, drift {drift[row['cfl']]:.3e} "
-              f"(published {row['drift']:.2e})", flush=True)
+        print(f"cfl {row['cfl']}: E0 at t = {rows[0]['time']:.1f}, t = {rows[-1]['time']:.1f}, "
+              f"drift {drift[row['cfl']]:.3e} (published {row['drift']:.2e})", flush=True)
         assert rows[-1]["time"] == pytest.approx(450.0, abs=0.5)
-    assert drift[0.2] < 0 and drift[0.5] < 0
-    assert abs(drift[0.5]) > 4 * abs(drift[0.2])
+        assert rows[0]["time"] == pytest.approx(90.0, abs=0.5)
+        assert abs(drift[row["cfl"]] / row["drift"] - 1.0) < 0.2
+    ratio = drift[0.5] / drift[0.2]
+    print(f"cfl 0.5 / cfl 0.2 drift: {ratio:.1f}x (published 13x)", flush=True)
+    assert 10.0 < ratio < 16.0
 
 
 @pytest.mark.parametrize("row", PUB["seed_threshold_128"]["rows"], ids=lambda r: f"eps{r['eps_ratio']}")
