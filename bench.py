@@ -145,6 +145,8 @@ def cpu_baseline(bs, state, budget_s=12.0, single_budget_s=5.0):
     else the visible cores up to 16).  One thread beside it, on a 128^3 box
     of the same physics (a 512^3 step takes over a minute on one core)."""
     from oracle import cpu as orc
+    if state is None:
+        raise RuntimeError("no state of the headline box to step")
     field, fluid = state
     nz, ny, nx = field.shape[:3]
     model = bs.Model(eta=sweep_eta(bs))
@@ -589,7 +591,12 @@ def main():
     sim.advance(start)
     kernel_ms, _ = sim.advance_timed(args.steps)
     # the CPU baseline steps this box from here (the state at the window's end)
-    cpu_state = None if args.no_cpu else sim.download()
+    cpu_state = None
+    if not args.no_cpu:
+        try:
+            cpu_state = sim.download()
+        except Exception as e:  # noqa: BLE001 -- the baseline then reports the failure
+            log(f"bench: the CPU baseline's state could not be read back: {type(e).__name__}: {e}")
     log(f"bench: steps {start}..{start + args.steps}: wall {wall_s * 1e3:.1f} ms, stage kernels "
         f"{kernel_ms:.1f} ms (second pass); broken {d.broken_fraction:.4f} max|v| {d.max_velocity:.4f} "
         f"nonfinite {d.nonfinite}")
