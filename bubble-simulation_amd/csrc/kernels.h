@@ -625,7 +625,7 @@ __device__ __forceinline__ void fused_tile(FusedLds<TX, TY, R> &L, const Params<
 
 // Minimum waves per SIMD the register allocator must leave room for: the
 // fast f32 stage, with its march unrolled by three (so the column rings
-// advance by register renaming), needs 116-124 VGPRs on the tiles that
+// advance by register renaming), needs 117-127 VGPRs on the tiles that
 // carry one halo item per thread (every fast tile), which 4 waves allow
 // without spilling (at 5 waves, 96 VGPRs, it spills); it runs about 0.5 %
 // faster than the rolled march at 5 waves.  The exact and f64
