@@ -406,20 +406,31 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99, flags=0):
     blat = np.array([r.latency_s for r in bres])
     burst = work / bwall
     ok = all(r.status == 0 for r in res) and all(r.status == 0 for r in bres)
-    seq_t, seq_work = 0.0, 0
+    seq_t, seq_work, service = 0.0, 0, []
     for (_, spec, steps) in jobs:
         sim = bs.Simulation(spec)
         sim.synchronize()
         t1 = time.perf_counter()
         sim.advance(steps)
         sim.synchronize()
-        seq_t += time.perf_counter() - t1
+        service.append(time.perf_counter() - t1)
+        seq_t += service[-1]
         seq_work += sim.cell_count() * steps
         sim.close()
     sequential = seq_work / seq_t
+    # the same arrivals served one job at a time, in arrival order, each for
+    # its measured plain-solve time: the latencies a sequential server gives
+    # at the same offered load
+    free, seq_lat = 0.0, []
+    for (arrival, _, _), t in zip(jobs, service):
+        start = max(arrival, free)
+        free = start + t
+        seq_lat.append(free - arrival)
+    seq_lat = np.array(seq_lat)
     log(f"bench: served {n_jobs} jobs at {rate_hz} Hz: wall {wall:.1f} s, {served / 1e9:.2f} G/s, "
         f"p50 {np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s; all at once: wall {bwall:.1f} s, "
-        f"{burst / 1e9:.2f} G/s; sequential {sequential / 1e9:.2f} G/s")
+        f"{burst / 1e9:.2f} G/s; sequential {sequential / 1e9:.2f} G/s, at the same arrivals one job at a time "
+        f"p50 {np.percentile(seq_lat, 50):.2f} s p99 {np.percentile(seq_lat, 99):.2f} s, last done at {free:.1f} s")
     acc = served_accuracy(bs, jobs, keep, kept, res, flags) if len(kept) == len(keep) else None
     if acc:
         log("bench: served accuracy (sample edges " + ", ".join(str(j["edge"]) for j in acc["jobs"]) + "): rel L2 "
@@ -430,6 +441,10 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99, flags=0):
             "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,
             "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": ok,
             "graphs": not (flags & bs.SERVER_NO_GRAPHS),
+            "sequential_fifo": {"p50_latency_s": float(np.percentile(seq_lat, 50)),
+                                "p99_latency_s": float(np.percentile(seq_lat, 99)), "wall_s": free,
+                                "note": "the same arrival times, one job at a time in arrival order, each for its "
+                                        "measured plain-solve time"},
             "burst": {"wall_s": bwall, "site_updates_per_s": burst, "speedup_vs_sequential": burst / sequential,
                       "p50_latency_s": float(np.percentile(blat, 50)),
                       "p99_latency_s": float(np.percentile(blat, 99))},
