@@ -546,7 +546,10 @@ typedef struct bs_server_config {
     uint64_t hbm_budget_bytes;   /* 0: use (total device memory - reserve)     */
     uint64_t reserve_bytes;      /* kept free                                  */
     uint32_t max_active;         /* 0: unlimited                               */
-    uint32_t chunk_steps;        /* steps per scheduling tick (>= 1)           */
+    uint32_t chunk_steps;        /* steps per scheduling tick (>= 1) of jobs of
+                                  * 128^3 cells and more; smaller jobs step
+                                  * in a class of their own, 1 to 8 chunks
+                                  * per tick                                   */
     int32_t precision;           /* BS_PRECISION_* for every job               */
     int32_t math;                /* BS_MATH_*                                  */
     uint32_t flags;              /* BS_SERVER_*                                */

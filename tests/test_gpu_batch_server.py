@@ -136,6 +136,28 @@ def test_server_alone_equals_mixed(bs, gpu_available):
     srv.close()
 
 
+def test_server_size_classes(bs, gpu_available):
+    # jobs below 128^3 cells step in a class of their own, more steps per
+    # tick while large jobs run: with equal step counts the small jobs of a
+    # mix finish before the large one (in one lockstep set all four would
+    # finish in the same tick), and every job still gives its solo bits
+    small = [bs.make_spec(grid=32, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, seed=k + 1) for k in range(3)]
+    large = bs.make_spec(grid=128, nucleation=bs.NUC_SIMULTANEOUS, bubbles=2, seed=9)
+    steps = 400
+    srv = bs.Server(budget_bytes=4 << 30, chunk_steps=10)
+    alone = [srv.wait(srv.submit(s, steps, checksum=True), 300).state_checksum for s in small + [large]]
+    ids = [srv.submit(s, steps, checksum=True) for s in [large] + small]
+    res = _wait_all(srv, ids)
+    assert all(res[j].status == 0 and res[j].steps == steps for j in ids)
+    assert [res[j].state_checksum for j in ids] == [alone[3]] + alone[:3]
+    t_large = res[ids[0]].finish_s
+    t_small = max(res[j].finish_s for j in ids[1:])
+    print(f"small jobs done at {t_small - res[ids[0]].submit_s:.3f} s, large at {t_large - res[ids[0]].submit_s:.3f} s",
+          flush=True)
+    assert t_small < t_large
+    srv.close()
+
+
 def test_server_without_graphs_equals_graphs(bs, gpu_available):
     # SERVER_NO_GRAPHS (eager stepping, for runs under a profiler) launches the
     # same kernels as the captured graphs: the same bits per job
