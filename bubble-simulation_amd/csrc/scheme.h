@@ -217,13 +217,26 @@ struct GammaLawEos {
     __host__ __device__ static inline R pressure_from_conserved(R e_lab, R z2, R /*phi*/, const Params<R> &P) {
         const R gm1 = P.eos_gamma - R(1);
         R p = gm1 * e_lab;
-        for (int it = 0; it < kIters; ++it) {
-            const R s = e_lab + p;
-            const R rs = M::div(R(1), s);
-            const R zs = z2 * rs;                        // Z^2 / s
-            const R f = p - gm1 * (e_lab - zs);
-            const R df = R(1) - gm1 * (zs * rs);         // 1 - (Gamma - 1) Z^2 / s^2
-            p = p - M::div(f, df);
+        if constexpr (M::fast) {
+            // the same Newton step with f and f' scaled by s^2, one
+            // reciprocal instead of two: f / f' = s (p s - (Gamma - 1)(E s -
+            // Z^2)) / (s^2 - (Gamma - 1) Z^2), the denominator > 0 since
+            // (Gamma - 1) Z^2 < s^2
+            const R gz2 = gm1 * z2;
+            for (int it = 0; it < kIters; ++it) {
+                const R s = e_lab + p;
+                const R fs = p * s - gm1 * (e_lab * s - z2);
+                p = p - (s * fs) * M::div(R(1), s * s - gz2);
+            }
+        } else {
+            for (int it = 0; it < kIters; ++it) {
+                const R s = e_lab + p;
+                const R rs = M::div(R(1), s);
+                const R zs = z2 * rs;                        // Z^2 / s
+                const R f = p - gm1 * (e_lab - zs);
+                const R df = R(1) - gm1 * (zs * rs);         // 1 - (Gamma - 1) Z^2 / s^2
+                p = p - M::div(f, df);
+            }
         }
         return M::max(p, P.p_floor);
     }
