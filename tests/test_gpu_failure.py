@@ -69,9 +69,11 @@ def test_batch_contains_a_nonfinite_box(bs, gpu_available):
 
 
 def test_server_evicts_a_nonfinite_job(bs, gpu_available):
+    # the 128^3 job steps in the large size class, the others and the
+    # poisoned one in the small class: the eviction happens with both stepping
     specs = [bs.make_spec(grid=g, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1, seed=k + 1)
-             for k, g in enumerate([(32, 32, 32), (48, 32, 40), (40, 40, 24)])]
-    steps = [60, 45, 70]
+             for k, g in enumerate([(32, 32, 32), (48, 32, 40), (40, 40, 24), (128, 128, 128)])]
+    steps = [60, 45, 70, 40]
     srv = bs.Server(budget_bytes=1 << 30, chunk_steps=5)
     try:
         alone = []
@@ -97,6 +99,6 @@ def test_server_evicts_a_nonfinite_job(bs, gpu_available):
             assert r.status == bs.BS_OK and r.steps == steps[k]
             assert r.state_checksum == alone[k], f"job {k} differs beside the non-finite one"
         st = srv.stats()
-        assert st.nonfinite == 1 and st.failed == 1 and st.completed == 6
+        assert st.nonfinite == 1 and st.failed == 1 and st.completed == 8
     finally:
         srv.close()
