@@ -309,7 +309,14 @@ def batched_sweep(bs, steps=1000, report_at=200, use_graph=True):
     log(f"bench: sequential plain solves {sequential / 1e9:.2f} G/s")
     return {"boxes": len(specs), "cells": cells, "steps": steps, "site_updates_per_s": batched,
             "sequential_site_updates_per_s": sequential, "speedup_vs_sequential": batched / sequential,
+            "sequential_note": SEQUENTIAL_NOTE,
             "plan": plan, "graph_replayed_steps": graph_steps, "accuracy": acc}
+
+
+SEQUENTIAL_NOTE = ("each box or job alone on the plain Simulation path, one after another, whole; that path "
+                   "gives boxes too small to fill the device short z chunks (2 planes at 32^3-64^3, 4 at 96^3, "
+                   "8 at 128^3) and 16x16 tiles where the x edge is an odd multiple of 16, so small boxes run "
+                   "at 1.4-19 G site-updates/s alone rather than latency-bound")
 
 
 def served_jobs(bs, n_jobs, rate_hz, seed):
@@ -439,6 +446,7 @@ def served_stream(bs, n_jobs=48, rate_hz=6.0, seed=99, flags=0):
     return {"jobs": n_jobs, "arrival_rate_hz": rate_hz, "budget_bytes": int(st.budget_bytes),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "site_updates_per_s": served, "sequential_site_updates_per_s": sequential,
+            "sequential_note": SEQUENTIAL_NOTE,
             "speedup_vs_sequential": served / sequential, "wall_s": wall, "all_ok": ok,
             "graphs": not (flags & bs.SERVER_NO_GRAPHS),
             "sequential_fifo": {"p50_latency_s": float(np.percentile(seq_lat, 50)),
