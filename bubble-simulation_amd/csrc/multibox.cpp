@@ -141,11 +141,13 @@ template <class R> static void fill_boxes(const std::vector<Entry *> &es, unsign
 // over every box of that EOS, padded with empty tasks to a multiple of 8,
 // so that the kernel can hand each XCD a contiguous run of tiles (blocks go
 // round-robin over the 8 XCDs; neighbouring tiles then share that XCD's L2)
-static void build_tasks(const std::vector<Entry *> &es, TileShape tile, int zc, int eos, std::vector<Task> &tasks) {
+static void build_tasks(const std::vector<Entry *> &es, TileShape tile, int plan_zc, int cls, int eos,
+                        std::vector<Task> &tasks) {
     tasks.clear();
     for (size_t i = 0; i < es.size(); ++i) {
         if (es[i]->spec.eos != eos) continue;
         const uint32_t *g = es[i]->spec.grid;
+        const int zc = es[i]->zc[cls] > 0 ? es[i]->zc[cls] : plan_zc;
         for (uint32_t zb = 0; zb < g[2]; zb += (uint32_t)zc)
             for (uint32_t y0 = 0; y0 < g[1]; y0 += (uint32_t)tile.ty)
                 for (uint32_t x0 = 0; x0 < g[0]; x0 += (uint32_t)tile.tx) {
@@ -174,7 +176,7 @@ int Engine::upload() {
     bool grow_tasks = false;
     for (int p = 0; p < 2; ++p)
         for (int c = 0; c < kEosClasses; ++c) {
-            build_tasks(entries, eos_tile(c, plan[p].tile), plan[p].zc, c, tasks[p][c]);
+            build_tasks(entries, eos_tile(c, plan[p].tile), plan[p].zc, p, c, tasks[p][c]);
             plan[p].ntasks[c] = (int)tasks[p][c].size();
             grow_tasks = grow_tasks || (size_t)plan[p].ntasks[c] > plan[p].cap[c];
         }

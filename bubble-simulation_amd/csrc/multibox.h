@@ -20,6 +20,10 @@ struct Entry {
     uint64_t *d_steps = nullptr;    // event stamping steps
     float *d_bubbles = nullptr;     // 4 floats per event
     uint64_t steps_target = UINT64_MAX;
+    // z chunk of the box's tasks per stage class ([0] RK stage 1, [1] stages
+    // 2 and 3); 0 takes the plan's.  A chunk fixed by the box alone keeps
+    // its bits the same in any set.
+    int zc[2] = {0, 0};
     // host mirrors of the device progress
     uint64_t steps_done = 0;
     int next_event = 0;

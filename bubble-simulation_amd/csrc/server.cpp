@@ -384,6 +384,19 @@ static void worker_loop(bs_server *s) {
                 s->n_active += 1;
             }
             Group &g = s->groups[j->group];
+            if (j->group == 0) {
+                // a small job's z chunks follow its own size, as on the plain
+                // path (a set of a few small boxes is bound by the latency of
+                // the march, not by the device), but no shorter than 8 planes,
+                // where the prologue would cost more than the latency it
+                // saves once the class holds several jobs.  At the 6 Hz sweep
+                // stream: p50 0.70 -> 0.56 s, p99 2.31 -> 2.05 s, the same
+                // capacity; chunks of 2-4 planes gave p50 0.62 s and lost 6 %
+                // of the capacity.
+                for (int p = 0; p < 2; ++p)
+                    j->entry->zc[p] =
+                        std::min(g.eng.plan[p].zc, std::max(8, default_zc(j->job.spec.grid, g.eng.plan[p].tile)));
+            }
             g.active.push_back(std::move(j));
             g.set_changed = true;
         }

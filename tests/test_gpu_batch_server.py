@@ -214,8 +214,10 @@ def test_server_overlaps_jobs(bs, gpu_available):
     # continuous batching: a job submitted while another runs is admitted and
     # advanced in the same launches, not queued behind it
     srv = bs.Server(budget_bytes=4 << 30, chunk_steps=5)
-    big = srv.submit(bs.make_spec(grid=96, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1), 400)
-    time.sleep(0.05)
+    # a job long enough (192^3, 300 steps: about 0.1 s) to be running when
+    # the second one arrives
+    big = srv.submit(bs.make_spec(grid=192, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1), 300)
+    time.sleep(0.02)
     small = srv.submit(bs.make_spec(grid=32, nucleation=bs.NUC_SIMULTANEOUS, bubbles=1), 40)
     rs = srv.wait(small, 300)
     rb = srv.wait(big, 300)
