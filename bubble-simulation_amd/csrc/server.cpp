@@ -22,9 +22,9 @@
 // diagnostics, reads back the state when the job asked for it (checksum,
 // keep_state), returns the device memory and publishes the result, while
 // the worker keeps the other jobs stepping.
-// A job's arithmetic depends only on its own spec: tasks never span boxes and
-// the tile shapes and z chunks are fixed for the server, so a job gives the
-// same bits alone and in any mix.
+// A job's arithmetic depends only on its own spec: tasks never span boxes,
+// the tile shapes are fixed for the server and a job's z chunks follow from
+// its grid, so a job gives the same bits alone and in any mix.
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -65,8 +65,9 @@ struct Job {
 // plain ratio of the cells, an equal share of the device, cut the median
 // latency of the 6 Hz sweep stream further, 0.95 -> 0.62 s, but nearly
 // doubled its 99th percentile; the square root takes it to 0.71 s at 2.30 s
-// against 2.19.)  The tile shapes and z chunks are the same in both
-// classes, so a job's bits still depend only on its own spec.
+// against 2.19.)  The tile shapes are the same in both classes and a small
+// job's z chunks follow from its own grid (below), so a job's bits still
+// depend only on its own spec.
 constexpr int kServerGroups = 2;
 constexpr uint64_t kSmallJobCells = uint64_t(1) << 21;  // below 128^3 cells: the small class
 constexpr uint64_t kSmallChunksMax = 8;
