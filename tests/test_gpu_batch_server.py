@@ -113,6 +113,53 @@ def test_batch_skips_the_stamp_only_when_nothing_is_due(bs, gpu_available):
     b.close()
 
 
+def test_zero_and_short_runs(bs, gpu_available):
+    # zero steps leave every path at the initialised lattice (the first
+    # step's nucleation stamp included: nothing is stamped before a step
+    # runs), runs shorter than a graph chunk or a server tick step eagerly
+    # to exactly their count, and a batch needs at least one box
+    specs = [bs.make_spec(grid=24, nucleation=bs.NUC_SIMULTANEOUS, bubbles=2, seed=3, math=bs.MATH_EXACT),
+             bs.make_spec(grid=(16, 24, 20), nucleation=bs.NUC_UNIFORM_STEPS, bubbles=2, seed=4, uniform_max_step=2,
+                          time_seed=9, math=bs.MATH_EXACT)]
+    ref0, ref3 = [], []
+    for s in specs:
+        sim = bs.Simulation(s)
+        init = sim.download()
+        sim.advance(0)
+        assert sim.steps == 0
+        got = sim.download()
+        assert np.array_equal(init[0], got[0]) and np.array_equal(init[1], got[1])
+        ref0.append(init)
+        sim.advance(3)
+        ref3.append(sim.download())
+        sim.close()
+    b = bs.Batch(specs)
+    b.advance(0, use_graph=True)
+    for i in range(len(specs)):
+        assert b.steps(i) == 0
+        f, u = b.download(i)
+        assert np.array_equal(f, ref0[i][0]) and np.array_equal(u, ref0[i][1]), f"box {i} after 0 steps"
+    b.advance(3, use_graph=True)
+    assert b.graph_steps() == 0
+    for i in range(len(specs)):
+        assert b.steps(i) == 3
+        f, u = b.download(i)
+        assert np.array_equal(f, ref3[i][0]) and np.array_equal(u, ref3[i][1]), f"box {i} after 3 steps"
+    b.close()
+    srv = bs.Server(budget_bytes=1 << 30, chunk_steps=5, math=bs.MATH_EXACT)
+    ids = [(srv.submit(s, n, keep_state=True), s, n) for s in specs for n in (0, 3)]
+    for j, s, n in ids:
+        r = srv.wait(j, 120)
+        assert r.status == 0 and r.steps == n
+        f, u = srv.fetch_state(j, s)
+        ref = (ref0 if n == 0 else ref3)[specs.index(s)]
+        assert np.array_equal(f, ref[0]) and np.array_equal(u, ref[1]), f"job of {n} steps"
+    srv.close()
+    with pytest.raises(bs.BubbleSimError) as e:
+        bs.Batch([])
+    assert e.value.code == bs.BS_ERR_INVALID
+
+
 def _wait_all(srv, ids, timeout=300):
     return {j: srv.wait(j, timeout) for j in ids}
 
