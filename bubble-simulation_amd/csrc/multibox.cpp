@@ -455,7 +455,7 @@ int autotune(Engine &eng) {
     }
     float best[2] = {1e30f, 1e30f};
     int best_t[2] = {3, 2}, best_zc[2] = {32, 32};
-    const int zcs[3] = {16, 32, 64};
+    const int zcs[4] = {16, 24, 32, 64};
     for (int ti = 0; ti < kNumTiles; ++ti) {
         if (!tile_built(kTiles[ti], math, eng.precision)) continue;
         if (eng.precision == BS_PRECISION_F64 && kTiles[ti].tx * kTiles[ti].ty > 256) continue;

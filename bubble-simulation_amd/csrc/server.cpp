@@ -499,7 +499,9 @@ int bs_server_create(const bs_server_config *cfg, bs_server **out) {
         // other two, which also stream U^n, on 32x8 (as the single box)
         g.eng.plan[0].tile = t0;
         g.eng.plan[1].tile = t1;
-        g.eng.plan[0].zc = g.eng.plan[1].zc = 32;
+        // 24-plane z chunks: the 48-job sweep stream all at once runs 0.5 %
+        // faster than with 32, at the same latencies under 6 Hz arrivals
+        g.eng.plan[0].zc = g.eng.plan[1].zc = 24;
     }
     rc = s->rs.alloc();
     if (rc) {

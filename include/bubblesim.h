@@ -512,7 +512,7 @@ int bs_batch_steps(const bs_batch *b, uint32_t box, uint64_t *steps_out);
 int bs_batch_tile(const bs_batch *b, uint32_t *tx, uint32_t *ty);
 /* The autotuner's choice for both classes of stage launch:
  * out = (tx, ty, zc) of RK stage 1, then (tx, ty, zc) of stages 2 and 3.
- * It times every built tile shape with z chunks of 16, 32 and 64 planes on
+ * It times every built tile shape with z chunks of 16, 24, 32 and 64 planes on
  * the batch's boxes with their bubbles stamped, before the first step. */
 int bs_batch_plan(const bs_batch *b, uint32_t out[6]);
 /* Steps advanced so far through replay of a captured hipGraph (25 steps per
