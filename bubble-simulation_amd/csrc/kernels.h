@@ -640,16 +640,25 @@ template <> struct Occupancy<FastMathF32> {
 // Single-box fused stage.  Blocks enumerate (tile_x, tile_y, z_chunk) with
 // tile_x fastest, in dispatch order: consecutive blocks land on different
 // XCDs (round-robin), so all eight XCDs stream neighbouring rows of the same
-// planes.  Giving each XCD a contiguous run of tiles instead (so that halo
-// re-reads hit its own L2) fetches 7 B/cell less in stages 2 and 3 but runs
-// them 4 % slower: the plain order keeps HBM streaming closer together.
+// planes.  The first RK stage remaps the tiles of each z chunk so that XCD
+// k takes the k-th eighth of them, a band of whole tile rows: x neighbours
+// then run on one XCD and their halo re-reads hit its L2, while the eight
+// XCDs still stream the same planes (at 512^3 its FETCH_SIZE per launch
+// drops from 7.90 to 4.15 GB, 1.29x the algorithmic read; the launch runs
+// 0.7-0.8 % faster from step 300 on, as fast before).  Stages 2 and 3,
+// which also stream U^n, keep the plain order: banded, they fetch 8.94 ->
+// 7.67 GB but run up to 1 % slower before step 300 and no faster after.
+// (A contiguous run of whole z chunks per XCD ran them 4 % slower.)
 template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc) {
     __shared__ FusedLds<TX, TY, R> L;
     const int b = blockIdx.x;
-    const int tx_i = b % ntx;
-    const int ty_i = (b / ntx) % nty;
-    const int zc_i = b / (ntx * nty);
+    const int T = ntx * nty;
+    const int zc_i = b / T;
+    int r = b - zc_i * T;                                       // tile within the chunk, dispatch order
+    if (S0 && (T & 7) == 0) r = (r & 7) * (T >> 3) + (r >> 3);  // XCD r & 7 takes the (r & 7)-th eighth
+    const int tx_i = r % ntx;
+    const int ty_i = r / ntx;
     const int zb = zc_i * zc;
     const int ze = min(zb + zc, box.P.nz);
     int in, out;
