@@ -645,10 +645,13 @@ template <> struct Occupancy<FastMathF32> {
 // then run on one XCD and their halo re-reads hit its L2, while the eight
 // XCDs still stream the same planes (at 512^3 its FETCH_SIZE per launch
 // drops from 7.90 to 4.15 GB, 1.29x the algorithmic read; the launch runs
-// 0.7-0.8 % faster from step 300 on, as fast before).  Stages 2 and 3,
-// which also stream U^n, keep the plain order: banded, they fetch 8.94 ->
-// 7.67 GB but run up to 1 % slower before step 300 and no faster after.
-// (A contiguous run of whole z chunks per XCD ran them 4 % slower.)
+// 0.7-0.8 % faster from step 300 on, as fast before).  Only tiles of 16
+// columns or fewer, whose x halo is the larger share of the fetch, are
+// banded: the gamma-law first stage on 32x8 tiles ran 0.2-0.4 % slower so.
+// Stages 2 and 3, which also stream U^n, keep the plain order: banded, they
+// fetch 8.94 -> 7.67 GB but run up to 1 % slower before step 300 and no
+// faster after.  (A contiguous run of whole z chunks per XCD ran them 4 %
+// slower.)
 template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc) {
     __shared__ FusedLds<TX, TY, R> L;
@@ -656,7 +659,7 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     const int T = ntx * nty;
     const int zc_i = b / T;
     int r = b - zc_i * T;                                       // tile within the chunk, dispatch order
-    if (S0 && (T & 7) == 0) r = (r & 7) * (T >> 3) + (r >> 3);  // XCD r & 7 takes the (r & 7)-th eighth
+    if (S0 && TX <= 16 && (T & 7) == 0) r = (r & 7) * (T >> 3) + (r >> 3);  // XCD r & 7 takes the (r & 7)-th eighth
     const int tx_i = r % ntx;
     const int ty_i = r / ntx;
     const int zb = zc_i * zc;
