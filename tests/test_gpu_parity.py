@@ -160,6 +160,25 @@ def test_every_tile_shape_matches_oracle(bs, orc, gpu_available, monkeypatch, ti
     assert np.array_equal(gf, of) and np.array_equal(gu, ou)
 
 
+@pytest.mark.parametrize("tile", ["16x16", "16x8"])
+def test_banded_first_stage_matches_oracle(bs, orc, gpu_available, monkeypatch, tile):
+    # 64 x 32 cells per plane give 8 (16x16) and 16 (16x8) tiles per z chunk,
+    # a multiple of 8, so the first RK stage hands each XCD a band of tile
+    # rows instead of dealing tiles round-robin: every tile must still be
+    # evaluated exactly once, so the state stays the oracle's bit for bit
+    monkeypatch.setenv("BS_TILE", tile)
+    monkeypatch.setenv("BS_ZC", "8")
+    grid = (64, 32, 20)
+    _log(f"case tile={tile} grid={grid}")
+    sched = _sched(bs, grid, [(0.0, 30.0, 16.0, 10.0), (0.4, 3.0, 30.0, 18.0)])
+    sim, o = _setup(bs, orc, grid, 0, bs.MATH_EXACT, bs.KERNEL_FUSED, sched)
+    sim.advance(4)
+    o.advance(4)
+    gf, gu = sim.download()
+    of, ou = o.state()
+    assert np.array_equal(gf, of) and np.array_equal(gu, ou)
+
+
 @pytest.mark.parametrize("grid", [(8, 8, 8), (8, 9, 10), (17, 8, 33)])
 def test_smallest_and_odd_boxes_match_oracle(bs, orc, gpu_available, grid):
     # boxes smaller than a tile wrap the tile onto itself (min edge 8, sim.rs:47)
