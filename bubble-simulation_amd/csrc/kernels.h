@@ -648,10 +648,15 @@ template <> struct Occupancy<FastMathF32> {
 // 0.7-0.8 % faster from step 300 on, as fast before).  Only tiles of 16
 // columns or fewer, whose x halo is the larger share of the fetch, are
 // banded: the gamma-law first stage on 32x8 tiles ran 0.2-0.4 % slower so.
-// Stages 2 and 3, which also stream U^n, keep the plain order: banded, they
-// fetch 8.94 -> 7.67 GB but run up to 1 % slower before step 300 and no
-// faster after.  (A contiguous run of whole z chunks per XCD ran them 4 %
-// slower.)
+// Stages 2 and 3, which also stream U^n, want the eight XCDs reading
+// neighbouring rows (banded, they fetch 8.94 -> 7.67 GB but run up to 1 %
+// slower before step 300; a contiguous run of whole z chunks per XCD ran
+// them 4 % slower): when the tile columns divide by 8, XCD k takes the k-th
+// vertical strip of ntx / 8 tile columns, and the eight walk their strips
+// row by row together, so that a tile's y neighbours and most of its x
+// neighbours share its L2 while the reads stay side by side (8.94 -> 7.88
+// GB per launch at 512^3, 0.1-0.6 % faster).  With one tile column per
+// strip this is the plain order.
 template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc) {
     __shared__ FusedLds<TX, TY, R> L;
@@ -660,8 +665,17 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     const int zc_i = b / T;
     int r = b - zc_i * T;                                       // tile within the chunk, dispatch order
     if (S0 && TX <= 16 && (T & 7) == 0) r = (r & 7) * (T >> 3) + (r >> 3);  // XCD r & 7 takes the (r & 7)-th eighth
-    const int tx_i = r % ntx;
-    const int ty_i = r / ntx;
+    int tx_i, ty_i;
+    if (!S0 && (ntx & 7) == 0) {
+        // XCD r & 7 takes a strip of ntx / 8 tile columns and walks it row
+        // by row, in step with the other seven
+        const int W = ntx >> 3, slot = r >> 3;
+        tx_i = (r & 7) * W + slot % W;
+        ty_i = slot / W;
+    } else {
+        tx_i = r % ntx;
+        ty_i = r / ntx;
+    }
     const int zb = zc_i * zc;
     const int ze = min(zb + zc, box.P.nz);
     int in, out;

@@ -160,17 +160,19 @@ def test_every_tile_shape_matches_oracle(bs, orc, gpu_available, monkeypatch, ti
     assert np.array_equal(gf, of) and np.array_equal(gu, ou)
 
 
-@pytest.mark.parametrize("tile", ["16x16", "16x8"])
-def test_banded_first_stage_matches_oracle(bs, orc, gpu_available, monkeypatch, tile):
-    # 64 x 32 cells per plane give 8 (16x16) and 16 (16x8) tiles per z chunk,
-    # a multiple of 8, so the first RK stage hands each XCD a band of tile
-    # rows instead of dealing tiles round-robin: every tile must still be
-    # evaluated exactly once, so the state stays the oracle's bit for bit
+@pytest.mark.parametrize("tile,grid", [("16x16", (64, 32, 20)), ("16x8", (64, 32, 20)), ("16x8", (256, 16, 12)),
+                                       ("32x8", (512, 8, 8))])
+def test_xcd_tile_maps_match_oracle(bs, orc, gpu_available, monkeypatch, tile, grid):
+    # tile counts per z chunk that divide by 8 switch the order in which
+    # tiles are dealt to the XCDs: the first RK stage on tiles of 16 columns
+    # takes bands of tile rows (8 and 16 tiles per chunk here), stages 2 and
+    # 3 vertical strips of ntx / 8 tile columns (2 columns on the 256- and
+    # 512-wide planes).  Every tile must still be evaluated exactly once, so
+    # the state stays the oracle's bit for bit
     monkeypatch.setenv("BS_TILE", tile)
     monkeypatch.setenv("BS_ZC", "8")
-    grid = (64, 32, 20)
     _log(f"case tile={tile} grid={grid}")
-    sched = _sched(bs, grid, [(0.0, 30.0, 16.0, 10.0), (0.4, 3.0, 30.0, 18.0)])
+    sched = _sched(bs, grid, [(0.0, grid[0] / 2 - 2, grid[1] / 2, grid[2] / 2), (0.4, 3.0, grid[1] - 2.0, 1.0)])
     sim, o = _setup(bs, orc, grid, 0, bs.MATH_EXACT, bs.KERNEL_FUSED, sched)
     sim.advance(4)
     o.advance(4)
