@@ -652,12 +652,13 @@ template <> struct Occupancy<FastMathF32> {
 // Stages 2 and 3, which also stream U^n, want the eight XCDs reading
 // neighbouring rows (banded, they fetch 8.94 -> 7.67 GB but run up to 1 %
 // slower before step 300; a contiguous run of whole z chunks per XCD ran
-// them 4 % slower): when the tile columns divide by 8, XCD k takes the k-th
-// vertical strip of ntx / 8 tile columns, and the eight walk their strips
-// row by row together, so that a tile's y neighbours and most of its x
-// neighbours share its L2 while the reads stay side by side (8.94 -> 7.88
-// GB per launch at 512^3, 0.1-0.6 % faster).  With one tile column per
-// strip this is the plain order.
+// them 4 % slower): when the tile columns divide by 4 and the tile rows by
+// 2, the chunk's tiles form a 4 x 2 grid of rectangles, one per XCD, and
+// the eight walk their rectangles row by row together, so that most of a
+// tile's x and y neighbours share its L2 while the XCDs read two bands of
+// rows at a time (8.94 -> 7.48 GB per launch at 512^3 and 0.3-0.6 %
+// faster, 384^3 2.2 % and 256^3 0.4-1.1 % faster; eight vertical strips
+// fetched 7.91 GB and gained 0.1-0.6 %, a 2 x 4 grid ran 3-4 % slower).
 template <int TX, int TY, class M, class R, bool S0, class E>
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occupancy<M>::waves))) k_stage_fused(Box<R> box, int stage, int ntx, int nty, int zc) {
     __shared__ FusedLds<TX, TY, R> L;
@@ -667,12 +668,13 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(Occ
     int r = b - zc_i * T;                                       // tile within the chunk, dispatch order
     if (S0 && TX <= 16 && (T & 7) == 0) r = (r & 7) * (T >> 3) + (r >> 3);  // XCD r & 7 takes the (r & 7)-th eighth
     int tx_i, ty_i;
-    if (!S0 && (ntx & 7) == 0) {
-        // XCD r & 7 takes a strip of ntx / 8 tile columns and walks it row
-        // by row, in step with the other seven
-        const int W = ntx >> 3, slot = r >> 3;
-        tx_i = (r & 7) * W + slot % W;
-        ty_i = slot / W;
+    if (!S0 && (ntx & 3) == 0 && (nty & 1) == 0) {
+        // the chunk's tiles as a 4 x 2 grid of rectangles: XCD k = r & 7
+        // takes rectangle (k % 4, k / 4) and walks it row by row, in step
+        // with the other seven
+        const int k = r & 7, slot = r >> 3, wx = ntx >> 2, hy = nty >> 1;
+        tx_i = (k & 3) * wx + slot % wx;
+        ty_i = (k >> 2) * hy + slot / wx;
     } else {
         tx_i = r % ntx;
         ty_i = r / ntx;

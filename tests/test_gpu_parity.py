@@ -161,14 +161,16 @@ def test_every_tile_shape_matches_oracle(bs, orc, gpu_available, monkeypatch, ti
 
 
 @pytest.mark.parametrize("tile,grid", [("16x16", (64, 32, 20)), ("16x8", (64, 32, 20)), ("16x8", (256, 16, 12)),
-                                       ("32x8", (512, 8, 8))])
+                                       ("32x8", (256, 32, 8)), ("32x8", (512, 8, 8))])
 def test_xcd_tile_maps_match_oracle(bs, orc, gpu_available, monkeypatch, tile, grid):
     # tile counts per z chunk that divide by 8 switch the order in which
     # tiles are dealt to the XCDs: the first RK stage on tiles of 16 columns
-    # takes bands of tile rows (8 and 16 tiles per chunk here), stages 2 and
-    # 3 vertical strips of ntx / 8 tile columns (2 columns on the 256- and
-    # 512-wide planes).  Every tile must still be evaluated exactly once, so
-    # the state stays the oracle's bit for bit
+    # takes bands of tile rows (8, 16 and 32 tiles per chunk here), stages 2
+    # and 3 a 4 x 2 grid of rectangles where the tile columns divide by 4 and
+    # the rows by 2 (1 x 2, 4 x 1 and 2 x 2 tiles per rectangle here; the
+    # single tile row of the 512-wide plane keeps the plain order).  Every
+    # tile must still be evaluated exactly once, so the state stays the
+    # oracle's bit for bit
     monkeypatch.setenv("BS_TILE", tile)
     monkeypatch.setenv("BS_ZC", "8")
     _log(f"case tile={tile} grid={grid}")
