@@ -648,7 +648,7 @@ template <> struct Occupancy<FastMathF32> {
 // 0.7-0.8 % faster from step 300 on, as fast before).  Only tiles of 16
 // columns or fewer, whose x halo is the larger share of the fetch, are
 // banded: the gamma-law first stage on 32x8 tiles ran 0.2-0.4 % slower so
-// (0.4-0.5 % in the strips below), and keeps the plain order.
+// (0.4-0.5 % in vertical strips), and keeps the plain order.
 // Stages 2 and 3, which also stream U^n, want the eight XCDs reading
 // neighbouring rows (banded, they fetch 8.94 -> 7.67 GB but run up to 1 %
 // slower before step 300; a contiguous run of whole z chunks per XCD ran
@@ -656,7 +656,7 @@ template <> struct Occupancy<FastMathF32> {
 // 2, the chunk's tiles form a 4 x 2 grid of rectangles, one per XCD, and
 // the eight walk their rectangles row by row together, so that most of a
 // tile's x and y neighbours share its L2 while the XCDs read two bands of
-// rows at a time (8.94 -> 7.48 GB per launch at 512^3 and 0.3-0.6 %
+// rows at a time (8.94 -> 7.48-7.55 GB per launch at 512^3 and 0.3-0.6 %
 // faster, 384^3 2.2 % and 256^3 0.4-1.1 % faster; eight vertical strips
 // fetched 7.91 GB and gained 0.1-0.6 %, a 2 x 4 grid ran 3-4 % slower).
 template <int TX, int TY, class M, class R, bool S0, class E>
