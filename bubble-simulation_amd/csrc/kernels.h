@@ -647,8 +647,9 @@ template <> struct Occupancy<FastMathF32> {
 // drops from 7.90 to 4.15 GB, 1.29x the algorithmic read; the launch runs
 // 0.7-0.8 % faster from step 300 on, as fast before).  Only tiles of 16
 // columns or fewer, whose x halo is the larger share of the fetch, are
-// banded: the gamma-law first stage on 32x8 tiles ran 0.2-0.4 % slower so
-// (0.4-0.5 % in vertical strips), and keeps the plain order.
+// banded: a first stage on 32x8 tiles (the ExactMath and f64 ones; measured
+// with the gamma-law EOS) ran 0.2-0.4 % slower so (0.4-0.5 % in vertical
+// strips), and keeps the plain order.
 // Stages 2 and 3, which also stream U^n, want the eight XCDs reading
 // neighbouring rows (banded, they fetch 8.94 -> 7.67 GB but run up to 1 %
 // slower before step 300; a contiguous run of whole z chunks per XCD ran

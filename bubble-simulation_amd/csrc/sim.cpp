@@ -50,11 +50,13 @@ int sim_alloc(bs_sim *s) {
     s->zc = default_zc(s->spec.grid, s->tile);
     // The first stage reads half the bytes of the other two and is bound by
     // instruction issue, where 16x16 tiles (fewer halo and far-edge faces per
-    // cell) run 5 % faster at 512^3; stages 2 and 3 are bound by HBM, where
-    // 32x8 fetches less.  An explicit BS_TILE applies to every stage.
-    const bool fast_bag = s->spec.math == BS_MATH_FAST && s->spec.precision == BS_PRECISION_F32 &&
-                          s->spec.eos == BS_EOS_BAG && s->spec.kernel == BS_KERNEL_FUSED;
-    rc = tile_from_env(fast_bag || odd16 ? TileShape{16, 16} : TileShape{32, 8}, math, s->spec.precision, &s->tile0);
+    // cell) run 5 % faster at 512^3 (the gamma-law EOS, whose halo cells
+    // each take eight Newton steps, 0.5-0.9 % on the whole step); stages 2
+    // and 3 are bound by HBM, where 32x8 fetches less.  An explicit BS_TILE
+    // applies to every stage.
+    const bool fast_f32 = s->spec.math == BS_MATH_FAST && s->spec.precision == BS_PRECISION_F32 &&
+                          s->spec.kernel == BS_KERNEL_FUSED;
+    rc = tile_from_env(fast_f32 || odd16 ? TileShape{16, 16} : TileShape{32, 8}, math, s->spec.precision, &s->tile0);
     if (rc) return rc;
     s->zc0 = default_zc(s->spec.grid, s->tile0);
     rc = s->bx.alloc(s->spec.precision, s->spec.grid, s->stream, false);
